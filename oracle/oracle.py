@@ -1,0 +1,282 @@
+"""CPU oracle for the k-NN search path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, and only to check or time; the product
+(``annvedi_amd``) never imports it and has no CPU fallback.
+
+Thin numpy/ctypes layer over ``oracle/liboracle.so`` (built from
+``oracle/oracle.cpp`` by ``oracle/Makefile``) plus numpy restatements of the
+evaluator's recall.  Reference file:line for each function is in the C++
+header and repeated below with what pins it.
+
+Parity pins (each a check a plausible mistake would fail):
+
+* ``linear_index_query`` (src/linear_index.cpp:27-121, src/simd.h:32-64):
+  golden ids/values written by the reference module compiled in this
+  container (tests/golden/linear_ref_*.npz), bit-exact; ranks 1-64 also equal
+  the FP64 exact oracle away from FP32 near-ties (tests/test_oracle.py).
+* ``l2_linear_index``: bit-exact against the compiled reference on
+  checkpoint-sensitive thresholds through the golden vectors; value equals
+  the plain float sum within 1e-5 relative; early-abandon returns a partial
+  that is >= threshold and <= the full value (invariants).
+* ``exact_topk_f64``: brute force in numpy float64 on tiny inputs, duplicate
+  rows (ties broken by lower id), k > n clamp, self-query at distance 0.
+* ``sq8_*`` (lsh_index_optimized.cpp:186-214, gpu_filter.cu:99-127): closed
+  forms -- rounding mode (half away vs half even) on planted .5 cases, the
+  constant-dimension conventions, clamp to [0, 255]; distance equals the
+  float64 dequantised distance within 1e-5 relative.
+* ``sq4_*`` (submission/src/hnsw.cpp:147-183, submission/src/simd.h:292-318):
+  nibble layout against a hand-built 32-dim block, padding inert, distance vs
+  float64 dequantised distance.
+* ``recall`` (orthogonal-competition/evaluator.py:149-157): hand-computed
+  cases including ties at the threshold.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_l2_linear_index.restype = ctypes.c_float
+        L.orc_l2_linear_index.argtypes = [_f32p, _f32p, _i64, ctypes.c_float]
+        L.orc_l2_linear_index_checkpoints.restype = _i32
+        L.orc_l2_linear_index_checkpoints.argtypes = [_f32p, _f32p, _i64, _f32p]
+        L.orc_linear_index_query.restype = _i32
+        L.orc_linear_index_query.argtypes = [_f32p, _i64, _i64, _f32p, _i32, _i64p, _f32p]
+        L.orc_linear_index_batch.restype = _i32
+        L.orc_linear_index_batch.argtypes = [_f32p, _i64, _i64, _f32p, _i32, _i32, _i64p, _f32p, _i32]
+        L.orc_l2_f64.restype = ctypes.c_double
+        L.orc_l2_f64.argtypes = [_f32p, _f32p, _i64]
+        L.orc_exact_topk_f64.restype = _i32
+        L.orc_exact_topk_f64.argtypes = [_f32p, _i64, _i64, _f32p, _i32, _i32, _i64p, _f64p, _i32]
+        L.orc_exact_topk_subset_f64.restype = _i32
+        L.orc_exact_topk_subset_f64.argtypes = [_f32p, _i64, _i64p, _i64, _f32p, _i32, _i64p, _f64p]
+        L.orc_f32_direct_l2.restype = ctypes.c_float
+        L.orc_f32_direct_l2.argtypes = [_f32p, _f32p, _i64]
+        L.orc_sq8_train.restype = None
+        L.orc_sq8_train.argtypes = [_f32p, _i64, _i64, _i32, _f32p, _f32p]
+        L.orc_sq8_encode.restype = None
+        L.orc_sq8_encode.argtypes = [_f32p, _i64, _i64, _i32, _f32p, _f32p, _u8p]
+        L.orc_sq8_l2.restype = ctypes.c_float
+        L.orc_sq8_l2.argtypes = [_u8p, _f32p, _f32p, _i64]
+        L.orc_sq8_l2_f64.restype = ctypes.c_double
+        L.orc_sq8_l2_f64.argtypes = [_u8p, _f32p, _f32p, _i64]
+        L.orc_sq4_train.restype = _i64
+        L.orc_sq4_train.argtypes = [_f32p, _i64, _i64, _f32p, _f32p]
+        L.orc_sq4_encode.restype = None
+        L.orc_sq4_encode.argtypes = [_f32p, _i64, _i64, _f32p, _f32p, _u8p]
+        L.orc_sq4_l2.restype = ctypes.c_float
+        L.orc_sq4_l2.argtypes = [_u8p, _f32p, _f32p, _i64]
+        L.orc_sq4_nibble.restype = _i32
+        L.orc_sq4_nibble.argtypes = [_u8p, _i64]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# --------------------------------------------------------------- linear ----
+def l2_linear_index(a, b, threshold=np.finfo(np.float32).max) -> np.float32:
+    a, b = _f32(a), _f32(b)
+    return np.float32(lib().orc_l2_linear_index(a, b, a.shape[0], float(threshold)))
+
+
+def l2_linear_index_checkpoints(a, b) -> np.ndarray:
+    a, b = _f32(a), _f32(b)
+    out = np.zeros(a.shape[0] // 96 + 2, dtype=np.float32)
+    n = lib().orc_l2_linear_index_checkpoints(a, b, a.shape[0], out)
+    return out[: n + 1]
+
+
+def linear_index_query(base, q, k):
+    base, q = _f32(base), _f32(q)
+    n, d = base.shape
+    ids = np.empty(k, dtype=np.int64)
+    vals = np.empty(k, dtype=np.float32)
+    got = lib().orc_linear_index_query(base, n, d, q, int(k), ids, vals)
+    return ids[:got], vals[:got]
+
+
+def linear_index_batch(base, Q, k, nthreads=1):
+    base, Q = _f32(base), _f32(np.atleast_2d(Q))
+    n, d = base.shape
+    nq = Q.shape[0]
+    ids = np.empty((nq, k), dtype=np.int64)
+    vals = np.empty((nq, k), dtype=np.float32)
+    got = lib().orc_linear_index_batch(base, n, d, Q, nq, int(k), ids, vals, int(nthreads))
+    return ids[:, :got], vals[:, :got]
+
+
+# ---------------------------------------------------------------- exact ----
+def l2_f64(a, b) -> float:
+    a, b = _f32(a), _f32(b)
+    return float(lib().orc_l2_f64(a, b, a.shape[0]))
+
+
+def exact_topk_f64(base, Q, k, nthreads=None):
+    """(ids int64 [nq,k'], d64 float64 [nq,k']) ranked by (d64, id); k' = min(k, n)."""
+    base, Q = _f32(base), _f32(np.atleast_2d(Q))
+    n, d = base.shape
+    nq = Q.shape[0]
+    if nthreads is None:
+        nthreads = max(1, min(8, len(os.sched_getaffinity(0))))
+    ids = np.empty((nq, k), dtype=np.int64)
+    dd = np.empty((nq, k), dtype=np.float64)
+    got = lib().orc_exact_topk_f64(base, n, d, Q, nq, int(k), ids, dd, int(nthreads))
+    return ids[:, :got], dd[:, :got]
+
+
+def exact_topk_subset_f64(base, cand, q, k):
+    base, q = _f32(base), _f32(q)
+    cand = np.ascontiguousarray(cand, dtype=np.int64)
+    ids = np.empty(k, dtype=np.int64)
+    dd = np.empty(k, dtype=np.float64)
+    got = lib().orc_exact_topk_subset_f64(base, base.shape[1], cand, cand.shape[0], q, int(k), ids, dd)
+    return ids[:got], dd[:got]
+
+
+def f32_direct_l2(a, b) -> np.float32:
+    a, b = _f32(a), _f32(b)
+    return np.float32(lib().orc_f32_direct_l2(a, b, a.shape[0]))
+
+
+# ------------------------------------------------------------------ SQ8 ----
+def sq8_train(x, variant=0):
+    x = _f32(x)
+    n, d = x.shape
+    s = np.empty(d, dtype=np.float32)
+    o = np.empty(d, dtype=np.float32)
+    lib().orc_sq8_train(x, n, d, int(variant), s, o)
+    return s, o
+
+
+def sq8_encode(x, scale, offset, variant=0):
+    x = _f32(x)
+    n, d = x.shape
+    codes = np.empty((n, d), dtype=np.uint8)
+    lib().orc_sq8_encode(x, n, d, int(variant), _f32(scale), _f32(offset), codes)
+    return codes
+
+
+def sq8_l2(code_row, q_shifted, scale) -> np.float32:
+    c = np.ascontiguousarray(code_row, dtype=np.uint8)
+    return np.float32(lib().orc_sq8_l2(c, _f32(q_shifted), _f32(scale), c.shape[0]))
+
+
+def sq8_l2_f64(code_row, q_shifted, scale) -> float:
+    c = np.ascontiguousarray(code_row, dtype=np.uint8)
+    return float(lib().orc_sq8_l2_f64(c, _f32(q_shifted), _f32(scale), c.shape[0]))
+
+
+def sq8_topk_f64(codes, scale, offset, Q, r):
+    """Top-r rows by float64 dequantised SQ8 distance, ties by lower id."""
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    Q = _f32(np.atleast_2d(Q))
+    deq = codes.astype(np.float64) * scale.astype(np.float64)[None, :]
+    out = []
+    for q in Q:
+        qs = (q.astype(np.float32) - offset).astype(np.float64)
+        dd = ((deq - qs[None, :]) ** 2).sum(axis=1)
+        order = np.lexsort((np.arange(dd.shape[0]), dd))[:r]
+        out.append(order)
+    return np.stack(out)
+
+
+# ------------------------------------------------------------------ SQ4 ----
+def sq4_train(x):
+    x = _f32(x)
+    n, d = x.shape
+    dp = (d + 31) // 32 * 32
+    s = np.empty(dp, dtype=np.float32)
+    o = np.empty(dp, dtype=np.float32)
+    lib().orc_sq4_train(x, n, d, s, o)
+    return s, o
+
+
+def sq4_encode(x, scale_pad, offset_pad):
+    x = _f32(x)
+    n, d = x.shape
+    dp = scale_pad.shape[0]
+    codes = np.empty((n, dp // 2), dtype=np.uint8)
+    lib().orc_sq4_encode(x, n, d, _f32(scale_pad), _f32(offset_pad), codes)
+    return codes
+
+
+def sq4_l2(code_row, q_shifted_pad, scale_pad) -> np.float32:
+    c = np.ascontiguousarray(code_row, dtype=np.uint8)
+    return np.float32(lib().orc_sq4_l2(c, _f32(q_shifted_pad), _f32(scale_pad), c.shape[0] * 2))
+
+
+def sq4_unpack(codes, dim):
+    """Nibbles as an n x dim uint8 array (inverse of the packing layout)."""
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    n, stride = codes.shape
+    out = np.empty((n, stride * 2), dtype=np.uint8)
+    blocks = codes.reshape(n, -1, 16)
+    out.reshape(n, -1, 32)[:, :, :16] = blocks & 0x0F
+    out.reshape(n, -1, 32)[:, :, 16:] = blocks >> 4
+    return out[:, :dim]
+
+
+# ---------------------------------------------------------------- recall ---
+def recall(true_dist, pred_dist, k):
+    """orthogonal-competition/evaluator.py:149-157: mean(pred[:k] <= true[k-1])."""
+    true_dist = np.asarray(true_dist)
+    pred_dist = np.asarray(pred_dist)
+    return np.array([np.mean(pred_dist[i][:k] <= true_dist[i][k - 1]) for i in range(true_dist.shape[0])])
+
+
+def euclid_f32(base, Q, ids):
+    """evaluator.py:455-460: float32 np.linalg.norm of (train[ids] - q)."""
+    base = np.asarray(base, dtype=np.float32)
+    Q = np.atleast_2d(np.asarray(Q, dtype=np.float32))
+    return np.stack([np.linalg.norm(base[ids[i]] - Q[i], axis=1) for i in range(Q.shape[0])])
+
+
+def recall_at_k(base, Q, pred_ids, true_ids, k):
+    """Evaluator-identical recall given predicted and ground-truth id lists."""
+    return recall(euclid_f32(base, Q, true_ids), euclid_f32(base, Q, pred_ids), k)
+
+
+# ------------------------------------------------------------ reference ----
+def ref_module(name):
+    """Import a reference module built by oracle/ref.mk into oracle/_ref, or None."""
+    import importlib
+    import sys
+
+    if not os.path.isdir(REF_DIR):
+        return None
+    if REF_DIR not in sys.path:
+        sys.path.insert(0, REF_DIR)
+    try:
+        return importlib.import_module(name)
+    except ImportError:
+        return None
