@@ -1,0 +1,83 @@
+"""The competition plugin surface on the MI355X search library.
+
+Mirrors the reference's ``Algorithm`` facade (algorithm.py:12-90) and its
+single-backend bundles (competitors/linear/algorithm.py:10-26,
+competitors/gpu_filter/algorithm.py:30-59): ``fit(train, **index_params)``,
+``query(q, k, **query_params) -> int64[k]`` and ``get_n_distances()``, plus
+``batch_query(Q, k, **query_params) -> (ids int64[B, k], dists float32[B, k])``.
+
+Selected in scenarios.yaml with ``backend: mi355x`` and ``mode`` in
+{exact, bf16, fp8, sq8, sq4}; ``rerank`` (alias ``r``, as in
+competitors/gpu_filter/scenarios.yaml) is the over-fetch for the quantized
+modes.  ``ds_size`` truncates the base as algorithm.py:19-21 does.
+
+``get_n_distances`` counts full-precision distance evaluations: N per query
+in exact mode (src/linear_index.cpp:108), the re-scored rows per query in the
+quantized modes (gpu_filter.cu:171, "only full Euclidean distances count").
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .index import Index
+
+BACKENDS = ("mi355x", "gpu", "linear")
+
+
+class Algorithm:
+    def __init__(self):
+        self._backend = "mi355x"
+        self._index = None
+        self._rerank = 0
+
+    def fit(self, train: np.ndarray, **index_params) -> None:
+        ds_size = index_params.pop("ds_size", None)
+        if ds_size is not None:
+            train = train[: int(ds_size)]
+        self._backend = str(index_params.pop("backend", "mi355x")).lower()
+        if self._backend not in BACKENDS:
+            raise ValueError(f"backend {self._backend!r} is not served by this plugin ({BACKENDS})")
+        # 'linear' keeps the reference's exact-scan meaning
+        mode = str(index_params.pop("mode", "exact")).lower()
+        self._rerank = int(index_params.pop("rerank", index_params.pop("r", 0)) or 0)
+        sq_variant = int(index_params.pop("sq_variant", 0))
+        max_batch = int(index_params.pop("max_batch", 4096))
+        if index_params:
+            raise ValueError(f"unknown index_params: {sorted(index_params)}")
+        self._index = Index(train, mode=mode, rerank=self._rerank, sq_variant=sq_variant, max_batch=max_batch)
+
+    def _need_index(self):
+        if self._index is None:
+            raise RuntimeError("index not trained: call fit() first")
+
+    def query(self, query: np.ndarray, k: int, **query_params) -> np.ndarray:
+        self._need_index()
+        q = np.asarray(query, dtype=np.float32)
+        if q.ndim != 1 or q.shape[0] != self._index.d:
+            raise RuntimeError(f"query must be a 1-D vector of length {self._index.d}")
+        r = int(query_params.get("rerank", query_params.get("r", self._rerank)) or 0)
+        ids, _ = self._index.search(q[None, :], int(k), rerank=r)
+        return ids[0]
+
+    def batch_query(self, queries, k: int, **query_params):
+        self._need_index()
+        r = int(query_params.get("rerank", query_params.get("r", self._rerank)) or 0)
+        if _is_device(queries):
+            return self._index.search_device(queries, int(k), rerank=r)
+        Q = np.asarray(queries, dtype=np.float32)
+        if Q.ndim != 2 or Q.shape[1] != self._index.d:
+            raise RuntimeError(f"queries must have shape (B, {self._index.d})")
+        return self._index.search(Q, int(k), rerank=r)
+
+    def get_n_distances(self) -> int:
+        if self._index is None:
+            return 0
+        return self._index.n_distances
+
+
+def _is_device(x) -> bool:
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(x, torch.Tensor) and x.is_cuda

@@ -1,0 +1,684 @@
+// C ABI of the k-NN search library (declared in include/annmi.h).
+//
+// Query pipeline per step of up to max_batch queries, all on the handle's
+// stream, one host synchronisation at the end:
+//   prep_queries                 scan operand, norms, exact-mode slack
+//   scan(rows [b0,b1)) with thr = +inf      every row of the first range
+//   select_threshold + scan(rows [b_j, b_j+1))  geometric ranges: the filter is
+//                                the rsel-th smallest candidate so far
+//   finalize                     collect, fp32 re-score, sort, fp64 near-ties
+//
+// Correctness of the filter: a row is dropped from a range only when its
+// approximate distance exceeds the rsel-th smallest approximate distance
+// among rows already scanned (plus, in exact mode, twice the rigorous error
+// bound of the approximation), so no row of the true top-k is ever dropped.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/annmi.h"
+#include "common.h"
+
+hipError_t launch_encode(int kind, const float* src, int64_t sstride, int64_t n, int d, int kdim, void* dst, float* xn,
+                         uint32_t* maxkey, float alpha, const float* scale, const float* offset, int variant,
+                         hipStream_t st);
+hipError_t launch_dim_minmax(const float* src, int64_t sstride, int64_t n, int d, uint32_t* mn, uint32_t* mx,
+                             hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st);
+hipError_t launch_prep_queries(int kind, const float* Q, int B, int d, int kdim, void* Qscan, float* qn, float* qscale,
+                               float* margin, float maxnorm, float alpha_x, const float* scale, const float* offset,
+                               hipStream_t st);
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+void set_err(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+#define HIP_OR(expr, ret)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            set_err("HIP error: %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, #expr); \
+            return ret;                                                                           \
+        }                                                                                         \
+    } while (0)
+
+constexpr int kMaxDim = 4096;
+constexpr int kCapDefault = 65536;
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int scan_kind_of(int mode) {
+    switch (mode) {
+        case ANNMI_EXACT_F32: return SCAN_F32;
+        case ANNMI_BF16: return SCAN_BF16;
+        case ANNMI_FP8: return SCAN_FP8;
+        case ANNMI_SQ8: return SCAN_SQ8;
+        case ANNMI_SQ4: return SCAN_SQ4;
+    }
+    return -1;
+}
+
+int chunk_of(int kind) {
+    switch (kind) {
+        case SCAN_F32: return 16;
+        case SCAN_BF16: return 32;
+        default: return 64;
+    }
+}
+
+int64_t scan_row_bytes(int kind, int kdim) {
+    switch (kind) {
+        case SCAN_F32: return (int64_t)kdim * 4;
+        case SCAN_BF16: return (int64_t)kdim * 2;
+        case SCAN_SQ4: return kdim / 2;
+        default: return kdim;
+    }
+}
+
+int64_t scan_q_bytes(int kind, int kdim) {
+    switch (kind) {
+        case SCAN_F32: return (int64_t)kdim * 4;
+        case SCAN_BF16: return (int64_t)kdim * 2;
+        default: return kdim;
+    }
+}
+
+struct Handle {
+    annmi_params p{};
+    int64_t n = 0;
+    int d = 0;
+    int kdim32 = 0;
+    float* X32 = nullptr;  // rerank base [n x kdim32]
+    bool own_x32 = false;
+    float* xn32 = nullptr;
+    float maxnorm = 0.f;
+    int kind = 0;
+    int kdim = 0;
+    void* Xs = nullptr;  // scan base
+    bool own_xs = false;
+    int64_t xs_stride = 0;
+    float* xn = nullptr;
+    float alpha_x = 1.f;
+    float* sq_scale = nullptr;
+    float* sq_offset = nullptr;
+    hipStream_t st = nullptr;
+    int64_t bytes = 0;
+    // workspace
+    int ws_B = 0;
+    int cap = kCapDefault;
+    float* dQ32 = nullptr;
+    void* dQs = nullptr;
+    float* qn = nullptr;
+    float* qscale = nullptr;
+    float* slack = nullptr;
+    float* thr = nullptr;
+    uint32_t* cnt = nullptr;
+    float* ckey = nullptr;
+    uint32_t* crid = nullptr;
+    uint32_t* ovf = nullptr;
+    uint32_t* nrr = nullptr;
+    int64_t* d_ids = nullptr;
+    float* d_dist = nullptr;
+    uint32_t* scratch_keys = nullptr;
+    std::vector<uint32_t> last_nrr, last_flags;
+    // optional per-kernel timing with HIP events on the handle's stream
+    bool prof = false;
+    double prof_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // scan launches, scan ms, scan bytes, scan flops, select ms, finalize ms, prep ms, steps
+    std::vector<hipEvent_t> ev_pool;
+    std::atomic<int64_t> ndist{0};
+    std::mutex mu;
+};
+
+void free_ws(Handle* h) {
+    dfree(h->dQ32);
+    dfree(h->dQs);
+    dfree(h->qn);
+    dfree(h->qscale);
+    dfree(h->slack);
+    dfree(h->thr);
+    dfree(h->cnt);
+    dfree(h->ckey);
+    dfree(h->crid);
+    dfree(h->ovf);
+    dfree(h->nrr);
+    dfree(h->d_ids);
+    dfree(h->d_dist);
+    h->dQ32 = nullptr;
+    h->dQs = nullptr;
+    h->qn = h->qscale = h->slack = h->thr = nullptr;
+    h->cnt = h->crid = h->ovf = h->nrr = nullptr;
+    h->ckey = nullptr;
+    h->d_ids = nullptr;
+    h->d_dist = nullptr;
+    h->ws_B = 0;
+}
+
+void free_handle(Handle* h) {
+    if (!h) return;
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    free_ws(h);
+    if (h->own_x32) dfree(h->X32);
+    dfree(h->xn32);
+    if (h->own_xs) dfree(h->Xs);
+    if (h->xn != h->xn32) dfree(h->xn);
+    dfree(h->sq_scale);
+    dfree(h->sq_offset);
+    dfree(h->scratch_keys);
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+}
+
+bool ensure_ws(Handle* h, int B, int k) {
+    if (h->ws_B >= B) return true;
+    free_ws(h);
+    const int64_t Bq = B;
+    HIP_OR(hipMalloc(&h->dQ32, Bq * h->d * sizeof(float)), false);
+    HIP_OR(hipMalloc(&h->dQs, Bq * scan_q_bytes(h->kind, h->kdim)), false);
+    HIP_OR(hipMalloc(&h->qn, Bq * sizeof(float)), false);
+    HIP_OR(hipMalloc(&h->qscale, Bq * sizeof(float)), false);
+    HIP_OR(hipMalloc(&h->slack, Bq * sizeof(float)), false);
+    HIP_OR(hipMalloc(&h->thr, Bq * sizeof(float)), false);
+    HIP_OR(hipMalloc(&h->cnt, Bq * sizeof(uint32_t)), false);
+    HIP_OR(hipMalloc(&h->ckey, Bq * h->cap * sizeof(float)), false);
+    HIP_OR(hipMalloc(&h->crid, Bq * h->cap * sizeof(uint32_t)), false);
+    HIP_OR(hipMalloc(&h->ovf, Bq * sizeof(uint32_t)), false);
+    HIP_OR(hipMalloc(&h->nrr, Bq * sizeof(uint32_t)), false);
+    HIP_OR(hipMalloc(&h->d_ids, Bq * 2048 * sizeof(int64_t)), false);
+    HIP_OR(hipMalloc(&h->d_dist, Bq * 2048 * sizeof(float)), false);
+    h->ws_B = B;
+    (void)k;
+    return true;
+}
+
+// Geometric row ranges [b_j, b_{j+1}): the first holds every row of the
+// first S0 rows, each later one grows by rho, so each adds about
+// rsel * (rho - 1) candidates on exchangeable data.
+std::vector<int64_t> range_bounds(int64_t n, int rsel, int cap) {
+    std::vector<int64_t> b{0};
+    int64_t s0 = std::max<int64_t>(4096, 8 * (int64_t)rsel);
+    s0 = std::min<int64_t>(s0, cap / 2);
+    s0 = std::max<int64_t>(s0, rsel);
+    int64_t rho = std::max<int64_t>(4, std::min<int64_t>(32, cap / (8 * (int64_t)rsel)));
+    int64_t cur = std::min<int64_t>(n, s0);
+    b.push_back(cur);
+    while (cur < n) {
+        cur = std::min<int64_t>(n, cur * rho);
+        b.push_back(cur);
+    }
+    return b;
+}
+
+// One step over <= ws_B queries already in h->dQ32; results in d_ids/d_dist.
+bool run_step(Handle* h, int B, int k, int R, int64_t* ids_out, float* d_out) {
+    const bool exact = h->kind == SCAN_F32;
+    const int kk = (int)std::min<int64_t>(k, h->n);
+    int rsel = exact ? kk : std::max(kk, R);
+    rsel = (int)std::min<int64_t>(rsel, h->n);
+    if (!exact && rsel > 2048) {
+        set_err("rerank R=%d exceeds the 2048-row re-score limit", rsel);
+        return false;
+    }
+    hipStream_t st = h->st;
+    // event slots: [0] step start, then per scan launch 2, select 2 each, finalize 2
+    std::vector<std::pair<int, int>> scan_ev, sel_ev;  // (start idx, end idx)
+    std::vector<double> scan_rows;
+    int ev_used = 0;
+    auto ev = [&](void) -> int {
+        if (!h->prof) return -1;
+        if (ev_used == (int)h->ev_pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            h->ev_pool.push_back(e);
+        }
+        int i = ev_used++;
+        (void)hipEventRecord(h->ev_pool[i], st);
+        return i;
+    };
+    int ev_prep0 = ev();
+    HIP_OR(launch_prep_queries(h->kind, h->dQ32, B, h->d, h->kdim, h->dQs, h->qn, h->qscale, h->slack, h->maxnorm,
+                               h->alpha_x, h->sq_scale, h->sq_offset, st),
+           false);
+    HIP_OR(launch_fill_u32(h->cnt, 0u, B, st), false);
+    HIP_OR(launch_fill_u32(h->ovf, 0u, B, st), false);
+    HIP_OR(launch_fill_f32(h->thr, __builtin_inff(), B, st), false);
+    CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
+    int ev_prep1 = ev();
+    std::vector<int64_t> bounds = range_bounds(h->n, rsel, h->cap);
+    for (size_t j = 0; j + 1 < bounds.size(); ++j) {
+        if (j > 0) {
+            SelectArgs sa{cb, B, rsel, h->slack, h->thr, h->ovf};
+            int e0 = ev();
+            HIP_OR(launch_select_threshold(sa, st), false);
+            int e1 = ev();
+            sel_ev.push_back({e0, e1});
+        }
+        ScanArgs a{};
+        a.kind = h->kind;
+        a.X = h->Xs;
+        a.xstride = h->xs_stride;
+        a.xn = h->xn;
+        a.row_begin = bounds[j];
+        a.row_end = bounds[j + 1];
+        a.d = h->d;
+        a.kdim = h->kdim;
+        a.Q = h->dQs;
+        a.qn = h->qn;
+        a.qscale = exact ? nullptr : h->qscale;
+        a.thr = h->thr;
+        a.B = B;
+        a.cand = cb;
+        int e0 = ev();
+        HIP_OR(launch_scan(a, st), false);
+        int e1 = ev();
+        scan_ev.push_back({e0, e1});
+        scan_rows.push_back((double)(a.row_end - a.row_begin));
+    }
+    FinalArgs fa{};
+    fa.cand = cb;
+    fa.B = B;
+    fa.k = k;
+    fa.rsel = rsel;
+    fa.slack = h->slack;
+    fa.X32 = h->X32;
+    fa.x32_stride = h->kdim32;
+    fa.d = h->d;
+    fa.Q32 = h->dQ32;
+    fa.out_ids = ids_out;
+    fa.out_d = d_out;
+    fa.overflow = h->ovf;
+    fa.n_reranked = h->nrr;
+    // fp32 re-score error bound (relative, both neighbours), doubled
+    const double u = 5.9604644775390625e-8;
+    fa.eps_rel = 4.0 * ((h->d + 15) / 16 + 28) * u;
+    int ev_f0 = ev();
+    HIP_OR(launch_finalize(fa, st), false);
+    int ev_f1 = ev();
+    std::vector<uint32_t> flags(B), nrr(B);
+    HIP_OR(hipMemcpyAsync(flags.data(), h->ovf, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
+    HIP_OR(hipMemcpyAsync(nrr.data(), h->nrr, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
+    HIP_OR(hipStreamSynchronize(st), false);
+    if (h->prof && ev_f1 >= 0) {
+        auto ms = [&](int a0, int a1) -> double {
+            float t = 0.f;
+            if (a0 < 0 || a1 < 0 || hipEventElapsedTime(&t, h->ev_pool[a0], h->ev_pool[a1]) != hipSuccess) return 0.0;
+            return (double)t;
+        };
+        const double row_bytes = (double)h->xs_stride;
+        for (size_t j = 0; j < scan_ev.size(); ++j) {
+            h->prof_acc[0] += 1;
+            h->prof_acc[1] += ms(scan_ev[j].first, scan_ev[j].second);
+            h->prof_acc[2] += scan_rows[j] * row_bytes + (double)B * scan_q_bytes(h->kind, h->kdim);
+            h->prof_acc[3] += 2.0 * (double)B * scan_rows[j] * (double)h->d;
+        }
+        for (auto& e : sel_ev) h->prof_acc[4] += ms(e.first, e.second);
+        h->prof_acc[5] += ms(ev_f0, ev_f1);
+        h->prof_acc[6] += ms(ev_prep0, ev_prep1);
+        h->prof_acc[7] += 1;
+    }
+    int64_t nd = 0;
+    for (int i = 0; i < B; ++i) {
+        if (flags[i] & 3u) {
+            set_err("query %d: candidate list overflow (flags=%u); rows are not in exchangeable order", i, flags[i]);
+            return false;
+        }
+        nd += exact ? h->n : (int64_t)nrr[i];
+    }
+    h->ndist += nd;
+    h->last_nrr.insert(h->last_nrr.end(), nrr.begin(), nrr.end());
+    h->last_flags.insert(h->last_flags.end(), flags.begin(), flags.end());
+    return true;
+}
+
+bool setup_mode(Handle* h) {
+    hipStream_t st = h->st;
+    const int64_t n = h->n;
+    const int d = h->d;
+    if (h->kind == SCAN_F32) {
+        h->kdim = h->kdim32;
+        h->Xs = h->X32;
+        h->xs_stride = (int64_t)h->kdim * 4;
+        h->xn = h->xn32;
+        return true;
+    }
+    h->kdim = round_up(d, chunk_of(h->kind));
+    h->xs_stride = scan_row_bytes(h->kind, h->kdim);
+    HIP_OR(hipMalloc(&h->Xs, n * h->xs_stride), false);
+    h->own_xs = true;
+    HIP_OR(hipMalloc(&h->xn, n * sizeof(float)), false);
+    h->bytes += n * h->xs_stride + n * 4;
+    float* sc = nullptr;
+    float* of = nullptr;
+    if (h->kind == SCAN_FP8 || h->kind == SCAN_SQ8 || h->kind == SCAN_SQ4) {
+        uint32_t *mn, *mx;
+        HIP_OR(hipMalloc(&mn, d * sizeof(uint32_t)), false);
+        HIP_OR(hipMalloc(&mx, d * sizeof(uint32_t)), false);
+        HIP_OR(launch_fill_u32(mn, 0xFFFFFFFFu, d, st), false);
+        HIP_OR(launch_fill_u32(mx, 0u, d, st), false);
+        HIP_OR(launch_dim_minmax(h->X32, h->kdim32, n, d, mn, mx, st), false);
+        std::vector<uint32_t> hmn(d), hmx(d);
+        HIP_OR(hipMemcpyAsync(hmn.data(), mn, d * 4, hipMemcpyDeviceToHost, st), false);
+        HIP_OR(hipMemcpyAsync(hmx.data(), mx, d * 4, hipMemcpyDeviceToHost, st), false);
+        HIP_OR(hipStreamSynchronize(st), false);
+        dfree(mn);
+        dfree(mx);
+        std::vector<float> fmn(d), fmx(d);
+        for (int j = 0; j < d; ++j) {
+            fmn[j] = key2f(hmn[j]);
+            fmx[j] = key2f(hmx[j]);
+        }
+        if (h->kind == SCAN_FP8) {
+            float m = 0.f;
+            for (int j = 0; j < d; ++j) m = std::max(m, std::max(std::fabs(fmn[j]), std::fabs(fmx[j])));
+            // power of two putting the largest |x| in [128, 256): e4m3 max is 448
+            float a = m > 0.f ? std::exp2(std::floor(std::log2(255.0f / m))) : 1.f;
+            while (m * a >= 256.f) a *= 0.5f;
+            h->alpha_x = a;
+        } else {
+            // scale/offset per the reference encoders (float arithmetic as there)
+            const int kd = h->kdim;
+            std::vector<float> hs(kd, 0.f), ho(kd, 0.f);
+            const float levels = h->kind == SCAN_SQ8 ? 255.0f : 15.0f;
+            for (int j = 0; j < d; ++j) {
+                if (fmx[j] - fmn[j] > 1e-8f) {
+                    hs[j] = (fmx[j] - fmn[j]) / levels;
+                    ho[j] = fmn[j];
+                } else {
+                    hs[j] = 1.0f;
+                    // SQ4 (submission/src/hnsw.cpp:166-168) and the gpu_filter SQ8
+                    // encoder keep offset = min; the ivf/hnsw SQ8 encoder uses 0.
+                    ho[j] = (h->kind == SCAN_SQ8 && h->p.sq_variant == 0) ? 0.0f : fmn[j];
+                }
+            }
+            HIP_OR(hipMalloc(&h->sq_scale, kd * sizeof(float)), false);
+            HIP_OR(hipMalloc(&h->sq_offset, kd * sizeof(float)), false);
+            HIP_OR(hipMemcpyAsync(h->sq_scale, hs.data(), kd * 4, hipMemcpyHostToDevice, st), false);
+            HIP_OR(hipMemcpyAsync(h->sq_offset, ho.data(), kd * 4, hipMemcpyHostToDevice, st), false);
+            HIP_OR(hipStreamSynchronize(st), false);
+            sc = h->sq_scale;
+            of = h->sq_offset;
+        }
+    }
+    HIP_OR(launch_encode(h->kind, h->X32, h->kdim32, n, d, h->kdim, h->Xs, h->xn, nullptr, h->alpha_x, sc, of,
+                         h->p.sq_variant, st),
+           false);
+    HIP_OR(hipStreamSynchronize(st), false);
+    return true;
+}
+
+Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params* params) {
+    g_err[0] = 0;
+    if (!base || n <= 0 || d <= 0) {
+        set_err("empty index: n=%lld d=%d", (long long)n, d);
+        return nullptr;
+    }
+    if (d > kMaxDim) {
+        set_err("dim %d > %d unsupported", d, kMaxDim);
+        return nullptr;
+    }
+    if (n > 0xFFFFFFF0ll) {
+        set_err("n=%lld exceeds the 32-bit row id range", (long long)n);
+        return nullptr;
+    }
+    int ndev = 0;
+    hipError_t derr = hipGetDeviceCount(&ndev);
+    if (derr != hipSuccess || ndev == 0) {
+        set_err("no HIP device visible (%s)", derr == hipSuccess ? "0 devices" : hipGetErrorString(derr));
+        return nullptr;
+    }
+    annmi_params p;
+    if (params)
+        p = *params;
+    else
+        annmi_default_params(&p, ANNMI_EXACT_F32);
+    int kind = scan_kind_of(p.mode);
+    if (kind < 0) {
+        set_err("unknown mode %d", p.mode);
+        return nullptr;
+    }
+    {
+        // the scan stages one 16-query tile in LDS (64 KB per workgroup)
+        const int kd = round_up(d, chunk_of(kind));
+        const int64_t lds = 16 * (scan_q_bytes(kind, kd) + 16);
+        if (lds > 65536) {
+            set_err("dim %d too large for mode %d (query tile needs %lld bytes of LDS)", d, p.mode, (long long)lds);
+            return nullptr;
+        }
+    }
+    Handle* h = new Handle();
+    h->p = p;
+    h->n = n;
+    h->d = d;
+    h->kind = kind;
+    h->kdim32 = round_up(d, 16);
+    if (p.max_batch <= 0) h->p.max_batch = 4096;
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        set_err("hipStreamCreate failed");
+        delete h;
+        return nullptr;
+    }
+    auto fail = [&]() -> Handle* {
+        std::string keep = g_err;
+        free_handle(h);
+        snprintf(g_err, sizeof(g_err), "%s", keep.c_str());
+        return nullptr;
+    };
+    // fp32 rows (re-score base; the exact scan base)
+    if (p.base_on_device && h->kdim32 == d && (reinterpret_cast<uintptr_t>(base) & 15) == 0) {
+        h->X32 = const_cast<float*>(base);
+        h->own_x32 = false;
+    } else {
+        if (hipMalloc(&h->X32, n * (int64_t)h->kdim32 * sizeof(float)) != hipSuccess) {
+            set_err("hipMalloc of %lld bytes for the fp32 rows failed", (long long)(n * (int64_t)h->kdim32 * 4));
+            return fail();
+        }
+        h->own_x32 = true;
+        h->bytes += n * (int64_t)h->kdim32 * 4;
+        hipMemcpyKind kindc = p.base_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        if (hipMemcpy2DAsync(h->X32, (size_t)h->kdim32 * 4, base, (size_t)d * 4, (size_t)d * 4, (size_t)n, kindc,
+                             h->st) != hipSuccess) {
+            set_err("copy of the base rows failed");
+            return fail();
+        }
+        if (h->kdim32 != d) {
+            // zero the padding columns
+            if (hipMemset2DAsync(h->X32 + d, (size_t)h->kdim32 * 4, 0, (size_t)(h->kdim32 - d) * 4, (size_t)n, h->st) !=
+                hipSuccess) {
+                set_err("padding clear failed");
+                return fail();
+            }
+        }
+    }
+    if (hipMalloc(&h->xn32, n * sizeof(float)) != hipSuccess) {
+        set_err("hipMalloc norms failed");
+        return fail();
+    }
+    h->bytes += n * 4;
+    uint32_t* mk = nullptr;
+    if (hipMalloc(&mk, sizeof(uint32_t)) != hipSuccess) return fail();
+    uint32_t zero = 0;
+    (void)hipMemcpyAsync(mk, &zero, 4, hipMemcpyHostToDevice, h->st);
+    if (launch_encode(SCAN_F32, h->X32, h->kdim32, n, d, h->kdim32, h->X32, h->xn32, mk, 1.f, nullptr, nullptr, 0,
+                      h->st) != hipSuccess) {
+        set_err("norm kernel launch failed");
+        dfree(mk);
+        return fail();
+    }
+    uint32_t hk = 0;
+    (void)hipMemcpyAsync(&hk, mk, 4, hipMemcpyDeviceToHost, h->st);
+    if (hipStreamSynchronize(h->st) != hipSuccess) {
+        set_err("index build failed: %s", hipGetErrorString(hipGetLastError()));
+        dfree(mk);
+        return fail();
+    }
+    dfree(mk);
+    h->maxnorm = key2f(hk);
+    if (!setup_mode(h)) return fail();
+    return h;
+}
+
+int32_t batch_impl(Handle* h, const float* Q, int32_t B, int32_t k, int32_t R, int64_t* ids, float* dists,
+                   bool device) {
+    g_err[0] = 0;
+    if (!h) {
+        set_err("null handle");
+        return -1;
+    }
+    if (B < 0 || k <= 0 || (B > 0 && (!Q || !ids || !dists))) {
+        set_err("bad arguments: B=%d k=%d", B, k);
+        return -1;
+    }
+    if (k > 2048) {
+        set_err("k=%d exceeds 2048", k);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    const int kk = (int)std::min<int64_t>(k, h->n);
+    if (R <= 0) R = h->p.rerank > 0 ? h->p.rerank : 4 * k;
+    h->last_nrr.clear();
+    h->last_flags.clear();
+    const int step = (int)std::min<int64_t>(h->p.max_batch, std::max(B, 1));
+    if (!ensure_ws(h, step, k)) return -1;
+    for (int s = 0; s < B; s += step) {
+        const int b = std::min(step, B - s);
+        hipMemcpyKind kin = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        HIP_OR(hipMemcpyAsync(h->dQ32, Q + (int64_t)s * h->d, (size_t)b * h->d * 4, kin, h->st), -1);
+        int64_t* oid = device ? ids + (int64_t)s * k : h->d_ids;
+        float* od = device ? dists + (int64_t)s * k : h->d_dist;
+        if (!run_step(h, b, k, R, oid, od)) return -1;
+        if (!device) {
+            HIP_OR(hipMemcpyAsync(ids + (int64_t)s * k, h->d_ids, (size_t)b * k * 8, hipMemcpyDeviceToHost, h->st), -1);
+            HIP_OR(hipMemcpyAsync(dists + (int64_t)s * k, h->d_dist, (size_t)b * k * 4, hipMemcpyDeviceToHost, h->st),
+                   -1);
+            HIP_OR(hipStreamSynchronize(h->st), -1);
+        }
+    }
+    return kk;
+}
+
+}  // namespace
+
+extern "C" {
+
+void annmi_default_params(annmi_params* p, int32_t mode) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->mode = mode;
+    p->rerank = 0;
+    p->sq_variant = 0;
+    p->base_on_device = 0;
+    p->max_batch = 4096;
+}
+
+void* annmi_create(const float* base, int64_t n, int32_t d, const annmi_params* params) {
+    return create_impl(base, n, d, params);
+}
+
+int32_t annmi_query(void* h, const float* q, int32_t k, int32_t rerank, int64_t* out_ids, float* out_dists) {
+    return batch_impl(static_cast<Handle*>(h), q, 1, k, rerank, out_ids, out_dists, false);
+}
+
+int32_t annmi_batch_query(void* h, const float* Q, int32_t B, int32_t k, int32_t rerank, int64_t* out_ids,
+                          float* out_dists) {
+    return batch_impl(static_cast<Handle*>(h), Q, B, k, rerank, out_ids, out_dists, false);
+}
+
+int32_t annmi_batch_query_device(void* h, const float* dQ, int32_t B, int32_t k, int32_t rerank, int64_t* d_ids,
+                                 float* d_dists) {
+    return batch_impl(static_cast<Handle*>(h), dQ, B, k, rerank, d_ids, d_dists, true);
+}
+
+int64_t annmi_ndist(void* h) { return h ? static_cast<Handle*>(h)->ndist.load() : 0; }
+
+int32_t annmi_last_batch_stats(void* hv, uint32_t* n_rescored, uint32_t* flags, int32_t max_q) {
+    Handle* h = static_cast<Handle*>(hv);
+    if (!h) {
+        set_err("null handle");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    int32_t m = (int32_t)std::min<size_t>(h->last_nrr.size(), (size_t)std::max(max_q, 0));
+    for (int i = 0; i < m; ++i) {
+        if (n_rescored) n_rescored[i] = h->last_nrr[i];
+        if (flags) flags[i] = h->last_flags[i];
+    }
+    return m;
+}
+
+void annmi_free(void* h) { free_handle(static_cast<Handle*>(h)); }
+
+const char* annmi_last_error(void) { return g_err; }
+
+int64_t annmi_index_bytes(void* h) { return h ? static_cast<Handle*>(h)->bytes : 0; }
+
+int32_t annmi_profile(void* hv, int32_t enable, double* out8) {
+    Handle* h = static_cast<Handle*>(hv);
+    if (!h) {
+        set_err("null handle");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (out8)
+        for (int i = 0; i < 8; ++i) out8[i] = h->prof_acc[i];
+    if (enable >= 0) {
+        h->prof = enable != 0;
+        for (int i = 0; i < 8; ++i) h->prof_acc[i] = 0;
+    }
+    return 8;
+}
+
+// ------------------------------------------------ gpu_filter drop-in ABI --
+void* gf_create(const float* train, long n, long dim) {
+    annmi_params p;
+    annmi_default_params(&p, ANNMI_SQ8);
+    p.sq_variant = 1;  // gpu_filter.cu:110-126 encoder
+    p.rerank = 500;    // competitors/gpu_filter/algorithm.py:43 default r
+    if (dim > kMaxDim) {
+        set_err("dim %ld > %d unsupported", dim, kMaxDim);
+        return nullptr;
+    }
+    return create_impl(train, (int64_t)n, (int32_t)dim, &p);
+}
+
+int gf_query(void* handle, const float* q, int k, int r, long long* out_ids) {
+    Handle* h = static_cast<Handle*>(handle);
+    if (!h) {
+        set_err("null handle");
+        return -1;
+    }
+    // gpu_filter.cu:145-146: r clamped to [k, n]
+    if (r < k) r = k;
+    if (r > h->n) r = (int)h->n;
+    std::vector<float> dd(k);
+    static_assert(sizeof(long long) == sizeof(int64_t), "");
+    return batch_impl(h, q, 1, k, r, reinterpret_cast<int64_t*>(out_ids), dd.data(), false);
+}
+
+long long gf_ndist(void* handle) { return handle ? (long long)static_cast<Handle*>(handle)->ndist.load() : 0; }
+
+void gf_free(void* handle) { free_handle(static_cast<Handle*>(handle)); }
+
+const char* gf_last_error(void) { return g_err; }
+
+}  // extern "C"

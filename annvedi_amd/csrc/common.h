@@ -1,0 +1,105 @@
+// Shared device/host definitions for the MI355X (gfx950) k-NN search library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+#define ANNMI_WAVE 64
+
+// Orderable unsigned image of a float: a < b (as floats, no NaN) iff
+// key(a) < key(b) (as unsigned).  Negative approximate distances (possible in
+// the ||x||^2 - 2x.q + ||q||^2 form) sort below zero.
+__host__ __device__ inline uint32_t f2key(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float key2f(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __builtin_bit_cast(float, u);
+}
+
+// Smallest float above f (f finite); +inf stays +inf.
+__host__ __device__ inline float next_up(float f) {
+    if (!(f < __builtin_inff())) return f;
+    return key2f(f2key(f) + 1u);
+}
+
+// Scan modes (element type of the candidate-generation pass).
+enum annmi_scan_kind : int {
+    SCAN_F32 = 0,
+    SCAN_BF16 = 1,
+    SCAN_FP8 = 2,
+    SCAN_SQ8 = 3,
+    SCAN_SQ4 = 4,
+};
+
+// Per-batch candidate workspace, struct-of-arrays, per query q:
+//   cnt[q]                 number of candidates appended (may exceed cap)
+//   key[q*cap + j]         approximate distance (float bits) of candidate j
+//   rid[q*cap + j]         row id of candidate j
+struct CandBuf {
+    uint32_t* cnt;
+    float* key;
+    uint32_t* rid;
+    int32_t cap;
+};
+
+// Per-query filter: a candidate passes when its approximate distance is
+// <= thr[q]; thr = tau + slack, computed by the select kernel.
+struct QueryPrep {
+    const float* qn;      // [B] squared norm of the query in the scan's element type
+    const float* thr;     // [B] current filter threshold
+};
+
+// ---------------------------------------------------------------- launches
+// All launch helpers return hipSuccess or the launch error; all run on `st`.
+struct ScanArgs {
+    int kind;                 // annmi_scan_kind
+    const void* X;            // base rows in the scan element type (row-major, row stride = xstride bytes)
+    int64_t xstride;          // bytes per row
+    const float* xn;          // [N] per-row squared norm term (scan element type)
+    int64_t row_begin, row_end;
+    int d;                    // logical dims
+    int kdim;                 // padded contraction length in elements (multiple of the MFMA K chunk)
+    const void* Q;            // [B x kdim] queries in the scan element type (row-major)
+    const float* qn;          // [B]
+    const float* qscale;      // [B] multiplier applied to the raw dot (fp8/sq8/sq4), or nullptr
+    const float* thr;         // [B]
+    int B;
+    CandBuf cand;
+};
+hipError_t launch_scan(const ScanArgs& a, hipStream_t st);
+
+
+struct SelectArgs {
+    CandBuf cand;
+    int B;
+    int rsel;                 // rank to select (k or rerank R)
+    const float* slack;       // [B] added to the selected key to form the threshold
+    float* thr;               // [B] out: new threshold (unchanged when cnt < rsel)
+    uint32_t* overflow;       // [B] out: set to 1 when cnt > cap
+};
+hipError_t launch_select_threshold(const SelectArgs& a, hipStream_t st);
+
+struct FinalArgs {
+    CandBuf cand;
+    int B;
+    int k;                    // output columns (rows past the candidate count get id -1)
+    int rsel;                 // k (exact) or R (over-fetch)
+    const float* slack;       // [B] collect slack (2*margin for exact mode, 0 otherwise)
+    const float* X32;         // fp32 base rows for the rerank
+    int64_t x32_stride;       // floats per row of X32
+    int d;
+    const float* Q32;         // [B x d] fp32 queries
+    int64_t* out_ids;         // [B x k]
+    float* out_d;             // [B x k]
+    uint32_t* overflow;       // [B] out flags: bit0 cand overflow, bit1 rerank truncated
+    uint32_t* n_reranked;     // [B] out: number of rows re-scored in fp32
+    double eps_rel;           // FP32 near-tie window (relative) for the FP64 boundary step
+};
+hipError_t launch_finalize(const FinalArgs& a, hipStream_t st);

@@ -1,0 +1,314 @@
+// Per-query selection over the candidate lists written by the scan:
+//
+//  * select_threshold -- k-th (or R-th) smallest approximate distance among
+//    a query's candidates so far, the filter for the next row range;
+//  * finalize -- pick the candidates to re-score, re-score them in fp32 in
+//    the reference's own summation order (src/simd.h:32-64: 16 lanes, lane l
+//    takes dims l, l+16, ... with FMA, tree reduction, 8-wide + scalar tail),
+//    sort by (distance, id), resolve fp32 near-ties in fp64, write top-k.
+//
+// The exact top-k rule being reproduced is src/linear_index.cpp:110-118
+// (partial_sort by (value, id): ties to the lower id) with the value taken
+// as the exact distance; gpu_filter.cu:161-174 is the over-fetch + rerank
+// shape.  The selection itself is a block-wide radix select on orderable
+// float keys: 11 + 11 + 10 bits, histograms in LDS.
+#include "common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kBins = 2048;
+constexpr int kRMax = 2048;   // most candidates one query re-scores
+
+// Exclusive scan helper: returns, for the bin histogram h[kBins] in LDS, the
+// bin b where the running count first reaches `target` (1-based rank) and the
+// count of keys in bins below b.  All threads get the result.
+__device__ void find_bin(const uint32_t* h, uint32_t target, uint32_t* sh, int* out_bin, uint32_t* out_below) {
+    // each thread owns 8 consecutive bins
+    const int t = threadIdx.x;
+    uint32_t local = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) local += h[t * 8 + j];
+    // inclusive scan across 256 threads (wave scan + cross-wave)
+    uint32_t v = local;
+    const int lane = t & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) sh[t >> 6] = v;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < (t >> 6); ++w) before += sh[w];
+    uint32_t incl = v + before;
+    uint32_t excl = incl - local;
+    if (excl < target && target <= incl) {
+        uint32_t c = excl;
+        for (int j = 0; j < 8; ++j) {
+            uint32_t hj = h[t * 8 + j];
+            if (c + hj >= target) {
+                sh[4] = (uint32_t)(t * 8 + j);
+                sh[5] = c;
+                break;
+            }
+            c += hj;
+        }
+    }
+    __syncthreads();
+    *out_bin = (int)sh[4];
+    *out_below = sh[5];
+    __syncthreads();
+}
+
+// Key of rank `rank` (1-based) among keys[0..n), n >= rank >= 1.
+__device__ uint32_t radix_select(const float* keys, int n, uint32_t rank, uint32_t* hist, uint32_t* sh) {
+    uint32_t prefix = 0, prefix_mask = 0;
+    const int shifts[3] = {21, 10, 0};
+    const int widths[3] = {11, 11, 10};
+    for (int p = 0; p < 3; ++p) {
+        for (int b = threadIdx.x; b < kBins; b += kThreads) hist[b] = 0;
+        __syncthreads();
+        const int sft = shifts[p];
+        const uint32_t wmask = (1u << widths[p]) - 1u;
+        for (int i = threadIdx.x; i < n; i += kThreads) {
+            uint32_t k = f2key(keys[i]);
+            if ((k & prefix_mask) == prefix) atomicAdd(&hist[(k >> sft) & wmask], 1u);
+        }
+        __syncthreads();
+        int bin;
+        uint32_t below;
+        find_bin(hist, rank, sh, &bin, &below);
+        rank -= below;
+        prefix |= (uint32_t)bin << sft;
+        prefix_mask |= wmask << sft;
+    }
+    return prefix;
+}
+
+__global__ __launch_bounds__(kThreads) void select_threshold(SelectArgs a) {
+    __shared__ uint32_t hist[kBins];
+    __shared__ uint32_t sh[8];
+    const int q = blockIdx.x;
+    uint32_t cnt = a.cand.cnt[q];
+    if (cnt > (uint32_t)a.cand.cap) {
+        if (threadIdx.x == 0) a.overflow[q] |= 1u;
+        cnt = (uint32_t)a.cand.cap;
+    }
+    if (cnt < (uint32_t)a.rsel) return;  // not enough rows seen yet: keep +inf
+    uint32_t k = radix_select(a.cand.key + (int64_t)q * a.cand.cap, (int)cnt, (uint32_t)a.rsel, hist, sh);
+    if (threadIdx.x == 0) {
+        float t = key2f(k) + a.slack[q];
+        // round the threshold up by one ulp so that the fp32 add never cuts a
+        // candidate whose exact sum would be admitted
+        t = next_up(t);
+        if (t < a.thr[q]) a.thr[q] = t;
+    }
+}
+
+// ---------------------------------------------------------------- finalize
+struct FinalSmem {
+    uint32_t hist[kBins];
+    uint32_t sh[8];
+    uint32_t ncol;
+    uint32_t id[kRMax];
+    float d32[kRMax];
+    uint64_t sk[kRMax];
+    uint8_t link[kRMax];
+};
+
+// fp32 squared distance of one row in the reference order, computed by a
+// 16-lane group; every lane of the group returns the value.
+__device__ float l2_ref_order(const float* x, const float* q, int d, int l16) {
+    const int d16 = d & ~15;
+    float s = 0.f;
+    for (int j = l16; j < d16; j += 16) {
+        float diff = x[j] - q[j];
+        s = fmaf(diff, diff, s);
+    }
+    // tree: h_j = s_j + s_{j+8}; (h0+h1), (h2+h3), ...; then pairs of pairs
+    s += __shfl_xor(s, 8, 16);
+    s += __shfl_xor(s, 1, 16);
+    s += __shfl_xor(s, 2, 16);
+    s += __shfl_xor(s, 4, 16);
+    // tail (d % 16 dims): 8-wide block when >= 8 remain, then scalar FMAs
+    int i = d16;
+    if (d - i >= 8) {
+        // separately rounded products and sums (no contraction into FMA)
+        float p[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            float diff = x[i + j] - q[i + j];
+            p[j] = __fmul_rn(diff, diff);
+        }
+        float g0 = __fadd_rn(p[0], p[4]), g1 = __fadd_rn(p[1], p[5]);
+        float g2 = __fadd_rn(p[2], p[6]), g3 = __fadd_rn(p[3], p[7]);
+        s = __fadd_rn(s, __fadd_rn(__fadd_rn(g0, g2), __fadd_rn(g1, g3)));
+        i += 8;
+    }
+    for (; i < d; ++i) {
+        float diff = x[i] - q[i];
+        s = fmaf(diff, diff, s);
+    }
+    return s;
+}
+
+__device__ double l2_f64(const float* x, const float* q, int d) {
+    double acc = 0.0;
+    for (int j = 0; j < d; ++j) {
+        double diff = (double)x[j] - (double)q[j];
+        double sq = __dmul_rn(diff, diff);
+        acc = __dadd_rn(acc, sq);
+    }
+    return acc;
+}
+
+// Bitonic sort of n (power of two) 64-bit keys in LDS, ascending.
+__device__ void bitonic_sort(uint64_t* v, int n) {
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < n / 2; i += kThreads) {
+                int lo = 2 * i - (i & (stride - 1));
+                int hi = lo + stride;
+                bool asc = (lo & size) == 0;
+                uint64_t a = v[lo], b = v[hi];
+                if ((a > b) == asc) {
+                    v[lo] = b;
+                    v[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void finalize(FinalArgs a) {
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    FinalSmem& S = *reinterpret_cast<FinalSmem*>(smem_raw);
+    float* qs = reinterpret_cast<float*>(smem_raw + sizeof(FinalSmem));
+    const int q = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    for (int j = threadIdx.x; j < a.d; j += kThreads) qs[j] = a.Q32[(int64_t)q * a.d + j];
+
+    uint32_t cnt = a.cand.cnt[q];
+    uint32_t flags = 0;
+    if (cnt > (uint32_t)a.cand.cap) {
+        flags |= 1u;
+        cnt = (uint32_t)a.cand.cap;
+    }
+    const float* keys = a.cand.key + (int64_t)q * a.cand.cap;
+    const uint32_t* rids = a.cand.rid + (int64_t)q * a.cand.cap;
+
+    // 1. collection threshold
+    float thr = __builtin_inff();
+    if (cnt > (uint32_t)a.rsel) {
+        uint32_t k = radix_select(keys, (int)cnt, (uint32_t)a.rsel, S.hist, S.sh);
+        thr = key2f(k);
+        if (a.slack[q] > 0.f) thr = next_up(thr + a.slack[q]);
+    }
+    // 2. collect candidates with key <= thr (ties at the pivot included)
+    if (threadIdx.x == 0) S.ncol = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)cnt; i += kThreads) {
+        if (keys[i] <= thr) {
+            uint32_t pos = atomicAdd(&S.ncol, 1u);
+            if (pos < (uint32_t)kRMax) S.id[pos] = rids[i];
+        }
+    }
+    __syncthreads();
+    uint32_t ncol = S.ncol;
+    if (ncol > (uint32_t)kRMax) {
+        flags |= 2u;
+        ncol = kRMax;
+    }
+    // 3. fp32 re-score, one row per 16-lane group
+    const int grp = threadIdx.x >> 4, l16 = threadIdx.x & 15;
+    for (uint32_t c = grp; c < ncol; c += kThreads / 16) {
+        const float* x = a.X32 + (int64_t)S.id[c] * a.x32_stride;
+        float v = l2_ref_order(x, qs, a.d, l16);
+        if (l16 == 0) S.d32[c] = v;
+    }
+    __syncthreads();
+    // 4. sort by (d32, id)
+    int np = 1;
+    while (np < (int)ncol) np <<= 1;
+    for (int i = threadIdx.x; i < np; i += kThreads)
+        S.sk[i] = i < (int)ncol ? (((uint64_t)f2key(S.d32[i]) << 32) | S.id[i]) : ~0ull;
+    __syncthreads();
+    bitonic_sort(S.sk, np);
+    // 5. fp64 boundary step: runs of neighbours closer than eps_rel (fp32
+    // error bound of both values) that start before rank k are re-scored in
+    // fp64 and re-ordered by (d64, id).
+    for (int i = threadIdx.x; i < (int)ncol; i += kThreads) {
+        uint8_t l = 0;
+        if (i + 1 < (int)ncol) {
+            float d0 = key2f((uint32_t)(S.sk[i] >> 32)), d1 = key2f((uint32_t)(S.sk[i + 1] >> 32));
+            l = (double)d1 - (double)d0 <= a.eps_rel * fabs((double)d1) ? 1 : 0;
+        }
+        S.link[i] = l;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)ncol && i < a.k; i += kThreads) {
+        if (!S.link[i] || (i > 0 && S.link[i - 1])) continue;  // not a run start
+        int e = i;
+        while (e + 1 < (int)ncol && S.link[e]) ++e;            // run [i, e]
+        // insertion sort of the run by (d64, id); at most a handful of rows
+        double dv[32];
+        uint64_t kv[32];
+        int m = e - i + 1;
+        if (m > 32) {
+            m = 32;
+            atomicOr(&a.overflow[q], 4u);  // run longer than the fp64 window: left in (fp32, id) order
+        }
+        for (int j = 0; j < m; ++j) {
+            kv[j] = S.sk[i + j];
+            dv[j] = l2_f64(a.X32 + (int64_t)(uint32_t)kv[j] * a.x32_stride, qs, a.d);
+        }
+        for (int j = 1; j < m; ++j) {
+            double dj = dv[j];
+            uint64_t kj = kv[j];
+            int t = j - 1;
+            while (t >= 0 && (dv[t] > dj || (dv[t] == dj && (uint32_t)kv[t] > (uint32_t)kj))) {
+                dv[t + 1] = dv[t];
+                kv[t + 1] = kv[t];
+                --t;
+            }
+            dv[t + 1] = dj;
+            kv[t + 1] = kj;
+        }
+        for (int j = 0; j < m; ++j) S.sk[i + j] = kv[j];
+    }
+    __syncthreads();
+    // 6. output
+    for (int j = threadIdx.x; j < a.k; j += kThreads) {
+        int64_t o = (int64_t)q * a.k + j;
+        if (j < (int)ncol) {
+            a.out_ids[o] = (int64_t)(uint32_t)S.sk[j];
+            a.out_d[o] = key2f((uint32_t)(S.sk[j] >> 32));
+        } else {
+            a.out_ids[o] = -1;
+            a.out_d[o] = __builtin_inff();
+        }
+    }
+    if (threadIdx.x == 0) {
+        a.overflow[q] |= flags;
+        a.n_reranked[q] = ncol;
+    }
+    (void)lane;
+}
+
+}  // namespace
+
+hipError_t launch_select_threshold(const SelectArgs& a, hipStream_t st) {
+    if (a.B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(select_threshold, dim3((unsigned)a.B), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+}
+
+size_t finalize_smem_bytes(int d) { return sizeof(FinalSmem) + (size_t)d * sizeof(float); }
+
+hipError_t launch_finalize(const FinalArgs& a, hipStream_t st) {
+    if (a.B <= 0) return hipSuccess;
+    size_t lds = finalize_smem_bytes(a.d);
+    hipLaunchKernelGGL(finalize, dim3((unsigned)a.B), dim3(kThreads), lds, st, a);
+    return hipGetLastError();
+}

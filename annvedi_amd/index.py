@@ -1,0 +1,133 @@
+"""Device-resident k-NN index over the C ABI (numpy and torch I/O).
+
+``Index`` owns one ``annmi`` handle.  Host arrays are copied to HBM at
+construction (the reference copies into a ``std::vector``,
+src/linear_index.cpp:81-82); a torch device tensor is borrowed instead and
+kept alive by the index.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class Index:
+    def __init__(self, base, mode: str = "exact", rerank: int = 0, sq_variant: int = 0, max_batch: int = 4096):
+        L = _lib.load()
+        if mode not in _lib.MODES:
+            raise ValueError(f"unknown mode {mode!r}; one of {sorted(_lib.MODES)}")
+        p = _lib.Params()
+        L.annmi_default_params(ctypes.byref(p), _lib.MODES[mode])
+        p.rerank = int(rerank)
+        p.sq_variant = int(sq_variant)
+        p.max_batch = int(max_batch)
+        self.mode = mode
+        self._keep = None
+        if _is_torch_cuda(base):
+            import torch
+
+            if base.dim() != 2:
+                raise RuntimeError("base must be a 2-D array")
+            base = base.contiguous().to(torch.float32)
+            torch.cuda.current_stream().synchronize()
+            p.base_on_device = 1
+            self._keep = base
+            n, d = base.shape
+            ptr = ctypes.c_void_p(base.data_ptr())
+        else:
+            arr = np.asarray(base)
+            if arr.ndim != 2:
+                raise RuntimeError("base must be a 2-D array")
+            arr = np.ascontiguousarray(arr, dtype=np.float32)
+            n, d = arr.shape
+            ptr = arr.ctypes.data_as(ctypes.c_void_p)
+        self.n, self.d = int(n), int(d)
+        h = L.annmi_create(ptr, self.n, self.d, ctypes.byref(p))
+        if not h:
+            raise RuntimeError(_lib.last_error())
+        self._h = ctypes.c_void_p(h)
+        self._L = L
+
+    # ---------------------------------------------------------------- queries
+    def _check_q(self, Q):
+        if Q.ndim != 2 or Q.shape[1] != self.d:
+            raise RuntimeError(f"queries must have shape (B, {self.d}), got {tuple(Q.shape)}")
+
+    def search(self, Q, k: int, rerank: int = 0):
+        """(ids int64 [B, k], dists float32 [B, k]) for a host batch."""
+        Q = np.ascontiguousarray(np.asarray(Q, dtype=np.float32))
+        if Q.ndim == 1:
+            Q = Q[None, :]
+        self._check_q(Q)
+        B = Q.shape[0]
+        ids = np.empty((B, k), dtype=np.int64)
+        dd = np.empty((B, k), dtype=np.float32)
+        got = self._L.annmi_batch_query(self._h, Q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), B, int(k),
+                                        int(rerank), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        dd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        if got < 0:
+            raise RuntimeError(_lib.last_error())
+        return ids[:, :got], dd[:, :got]
+
+    def search_device(self, Q, k: int, rerank: int = 0):
+        """Same for a torch device tensor; returns device tensors."""
+        import torch
+
+        self._check_q(Q)
+        Q = Q.contiguous().to(torch.float32)
+        B = Q.shape[0]
+        ids = torch.empty((B, k), dtype=torch.int64, device=Q.device)
+        dd = torch.empty((B, k), dtype=torch.float32, device=Q.device)
+        torch.cuda.current_stream().synchronize()
+        got = self._L.annmi_batch_query_device(self._h, ctypes.c_void_p(Q.data_ptr()), B, int(k), int(rerank),
+                                               ctypes.c_void_p(ids.data_ptr()), ctypes.c_void_p(dd.data_ptr()))
+        if got < 0:
+            raise RuntimeError(_lib.last_error())
+        return ids[:, :got], dd[:, :got]
+
+    def last_batch_stats(self, B: int):
+        n = np.zeros(B, dtype=np.uint32)
+        f = np.zeros(B, dtype=np.uint32)
+        got = self._L.annmi_last_batch_stats(self._h, n.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                             f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), B)
+        return n[:got], f[:got]
+
+    def profile(self, enable: int = -1) -> dict:
+        """Per-kernel event timing counters (see annmi_profile); enable=1 starts
+        a fresh measurement, 0 stops, -1 only reads."""
+        out = (ctypes.c_double * 8)()
+        if self._L.annmi_profile(self._h, int(enable), out) < 0:
+            raise RuntimeError(_lib.last_error())
+        keys = ["scan_launches", "scan_ms", "scan_bytes", "scan_flops", "select_ms", "finalize_ms", "prep_ms", "steps"]
+        return dict(zip(keys, list(out)))
+
+    @property
+    def n_distances(self) -> int:
+        return int(self._L.annmi_ndist(self._h))
+
+    @property
+    def nbytes(self) -> int:
+        return int(self._L.annmi_index_bytes(self._h))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.annmi_free(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _is_torch_cuda(x) -> bool:
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(x, torch.Tensor) and x.is_cuda

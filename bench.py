@@ -1,0 +1,272 @@
+"""Benchmark of the served k-NN path on one MI355X.
+
+Workload (default): 10,000,000 x 384 float32 Gaussian mixture (4096 clusters,
+sigma 0.3, L2-normalised) generated directly in HBM; query batches with sizes
+drawn log-uniform in [1, 4096]; k = 100.  A step is one batch through the
+path: candidate scan (bf16 MFMA by default) over all rows with fused
+threshold filtering, top-R selection, fp32 re-score of R rows, fp64
+boundary step, top-100.  R is calibrated before timing so that recall@100 on
+held-out queries is >= 0.99; recall is then measured on the timed queries
+against the plain fp32 exact scan, which is also timed on the same batches.
+
+Prints one JSON line (rank 0, N = 1).  Everything else goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "sq8": 5000.0, "sq4": 5000.0, "exact": 157.3}
+DTYPE = {"exact": "fp32", "bf16": "bf16", "fp8": "fp8", "sq8": "int8", "sq4": "int8"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def batch_sizes(n, lo=1, hi=4096, seed=4000):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi), size=n))).astype(int).clip(lo, hi)
+
+
+def run_batches(idx, Q, sizes, k, rerank=0):
+    """Times every batch (synchronous); returns per-batch seconds and outputs."""
+    import torch
+
+    out_ids, times = [], []
+    off = 0
+    for b in sizes:
+        q = Q[off:off + b]
+        off += b
+        t0 = time.perf_counter()
+        ids, _ = idx.search_device(q, k, rerank=rerank)
+        times.append(time.perf_counter() - t0)
+        out_ids.append(ids)
+    torch.cuda.synchronize()
+    return np.array(times), out_ids
+
+
+def recall_vs_exact(base, Q, pred_ids, true_ids, k):
+    """evaluator.py:149-157 recall with float32 Euclidean distances, on the GPU."""
+    import torch
+
+    def dist(ids):
+        rows = base[ids.reshape(-1)].view(ids.shape[0], ids.shape[1], -1)
+        return torch.linalg.vector_norm(rows - Q[:, None, :], dim=2)
+
+    td = dist(true_ids)
+    pd = dist(pred_ids)
+    thr = td[:, k - 1:k]
+    return (pd[:, :k] <= thr).float().mean(dim=1).cpu().numpy()
+
+
+def cpu_baseline(base_t, Q_t, budget_s=15.0, max_q=200):
+    """The reference CPU path (oracle/_ref/linear_ann_cpp, built from
+    src/linear_index.cpp) timed on this host: one thread, one query at a time,
+    on a 1M-row prefix of the base; QPS scaled by rows to the full base."""
+    from oracle import oracle as O
+
+    n_full = base_t.shape[0]
+    n_s = min(n_full, 1_000_000)
+    xb = base_t[:n_s].cpu().numpy()
+    xq = Q_t[:max_q].cpu().numpy()
+    ref = O.ref_module("linear_ann_cpp")
+    t_used = 0.0
+    nq = 0
+    if ref is not None:
+        idx = ref.LinearIndex()
+        idx.fit(xb)
+        kind = "reference"
+        while nq < xq.shape[0] and t_used < budget_s:
+            t0 = time.perf_counter()
+            idx.query(xq[nq], 100)
+            t_used += time.perf_counter() - t0
+            nq += 1
+    else:
+        kind = "port"
+        while nq < xq.shape[0] and t_used < budget_s:
+            t0 = time.perf_counter()
+            O.linear_index_query(xb, xq[nq], 100)
+            t_used += time.perf_counter() - t0
+            nq += 1
+    qps_sample = nq / t_used
+    multi = None
+    if ref is not None:
+        from concurrent.futures import ThreadPoolExecutor
+
+        cores = len(os.sched_getaffinity(0))
+        nmq = min(xq.shape[0], 4 * cores)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(cores) as ex:
+            list(ex.map(lambda i: idx.query(xq[i], 100), range(nmq)))
+        dt = time.perf_counter() - t0
+        multi = {"value": round(nmq / dt * n_s / n_full, 3), "cores": cores, "queries": nmq}
+    return {
+        "value": round(qps_sample * n_s / n_full, 4),
+        "unit": "queries/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"src/linear_index.cpp LinearIndex.query (k=100, 1 thread, one query per call) on the first "
+                  f"{n_s} rows x {xb.shape[1]} of the base, {nq} queries in {t_used:.1f}s; QPS scaled by "
+                  f"{n_s}/{n_full} rows to the full base",
+        "all_cores": multi,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="bf16", choices=["bf16", "fp8", "sq8", "sq4", "exact"])
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--d", type=int, default=384)
+    ap.add_argument("--k", type=int, default=100)
+    ap.add_argument("--rerank", type=int, default=0, help="fixed R (0: calibrate to recall >= 0.99)")
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain fp32 path")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(the index would shard by rows or queries; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from annvedi_amd import Index, data
+
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+    n, d, k = args.n, args.d, args.k
+    C, sigma = 4096, 0.3
+    log(f"[bench] generating {n} x {d} mixture in HBM")
+    base = data.mixture_torch(n, d, C, sigma, 2003, center_seed=1003, normalize=True, device=dev)
+    sizes = batch_sizes(args.warmup + args.steps)
+    nq_total = int(sizes.sum())
+    Q = data.mixture_torch(nq_total, d, C, sigma, 3003, center_seed=1003, normalize=True, device=dev)
+    Qcal = data.mixture_torch(512, d, C, sigma, 3999, center_seed=1003, normalize=True, device=dev)
+    torch.cuda.synchronize()
+    log(f"[bench] batches {sizes.tolist()} ({nq_total} queries)")
+
+    exact = Index(base, mode="exact")
+    t0 = time.perf_counter()
+    gt_cal, _ = exact.search_device(Qcal, k)
+    log(f"[bench] calibration ground truth: {time.perf_counter() - t0:.2f}s")
+
+    mode = args.mode
+    if mode == "exact":
+        idx, R = exact, 0
+    else:
+        idx = Index(base, mode=mode)
+        R = args.rerank
+        if R <= 0:
+            for R in (200, 300, 400, 600, 800, 1200, 1600, 2048):
+                ids, _ = idx.search_device(Qcal, k, rerank=R)
+                rc = float(recall_vs_exact(base, Qcal, ids, gt_cal, k).mean())
+                log(f"[bench] calibrate R={R}: recall@{k} = {rc:.4f}")
+                if rc >= 0.995:
+                    break
+    # warmup, then timed steps
+    wsz, tsz = sizes[:args.warmup], sizes[args.warmup:]
+    run_batches(idx, Q[: int(wsz.sum())], wsz, k, R)
+    Qt = Q[int(wsz.sum()):]
+    idx.profile(1)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    times, out_ids = run_batches(idx, Qt, tsz, k, R)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    prof = idx.profile(0)
+    nq = int(tsz.sum())
+    qps = nq / wall
+    lat = np.repeat(times, tsz) * 1e3
+    log(f"[bench] {mode}: {nq} queries in {wall:.3f}s -> {qps:.1f} QPS; prof {prof}")
+
+    # recall of the timed answers vs the exact scan (sample of <= 2048 queries)
+    pred = torch.cat(out_ids)
+    m = min(nq, 2048)
+    sel = torch.linspace(0, nq - 1, m, device=dev).long()
+    recall = None
+    plain = None
+    if mode != "exact":
+        gt, _ = exact.search_device(Qt[sel], k)
+        recall = float(recall_vs_exact(base, Qt[sel], pred[sel], gt, k).mean())
+        log(f"[bench] recall@{k} on {m} timed queries: {recall:.4f}")
+        if not args.no_plain:
+            exact.profile(1)
+            tp0 = time.perf_counter()
+            ptimes, _ = run_batches(exact, Qt, tsz, k)
+            pwall = time.perf_counter() - tp0
+            pprof = exact.profile(0)
+            plain = {"qps": round(nq / pwall, 2), "ms_per_step": round(pwall / len(tsz) * 1e3, 3),
+                     "p50_ms": round(float(np.percentile(np.repeat(ptimes, tsz) * 1e3, 50)), 3),
+                     "p99_ms": round(float(np.percentile(np.repeat(ptimes, tsz) * 1e3, 99)), 3),
+                     "scan_ms": round(pprof["scan_ms"], 2)}
+            log(f"[bench] plain fp32: {plain}")
+
+    # roofline of the dominant kernel (the scan), from in-library HIP events
+    scan_s = prof["scan_ms"] / 1e3
+    gbs = prof["scan_bytes"] / scan_s / 1e9 if scan_s > 0 else 0.0
+    tfl = prof["scan_flops"] / scan_s / 1e12 if scan_s > 0 else 0.0
+    peak_t = MFMA_PEAK_TFLOPS[mode]
+    f_hbm, f_mfma = gbs / HBM_PEAK_GBS, tfl / peak_t
+    if f_hbm >= f_mfma:
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(f_hbm, 4)}
+    else:
+        roof = {"bound": "mfma", "achieved": round(tfl, 2), "peak": peak_t, "unit": "TFLOP/s",
+                "frac": round(f_mfma, 4)}
+    roof["traffic"] = None
+    roof["kernel"] = f"scan_kernel<{mode}>"
+    roof["launches"] = int(prof["scan_launches"])
+    roof["avg_launch_ms"] = round(prof["scan_ms"] / max(prof["scan_launches"], 1), 4)
+    roof["other_frac"] = {"hbm": round(f_hbm, 4), "mfma": round(f_mfma, 4)}
+    roof["time_split_ms"] = {k2: round(prof[k2], 2) for k2 in ("scan_ms", "select_ms", "finalize_ms", "prep_ms")}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] timing the reference CPU path")
+        cpu = cpu_baseline(base, Qt)
+        log(f"[bench] cpu baseline {cpu}")
+
+    res = {
+        "metric": "QPS at recall@100>=0.99, 10Mx384, 1 MI355X vs plain FP32; p50/p99 ms",
+        "value": round(qps, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": int(args.steps),
+        "warmup": int(args.warmup),
+        "ms_per_step": round(wall / len(tsz) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[mode],
+        "data": "synthetic",
+        "config": {"workload": f"mixed-{n // 1_000_000}m-{d}" if n >= 1_000_000 else f"mixed-{n}-{d}",
+                   "n": n, "d": d, "clusters": C, "sigma": sigma, "normalized": True, "k": k,
+                   "batch": "log-uniform [1, 4096]", "queries": nq, "mode": mode, "rerank": R,
+                   "rerank_dtype": "fp32"},
+        "recall_at_100": recall,
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "plain_fp32": plain,
+        "speedup_vs_plain_fp32": round(qps / plain["qps"], 3) if plain else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

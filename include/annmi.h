@@ -1,0 +1,110 @@
+/*
+ * annmi -- MI355X (gfx950) k-NN search library: exact FP32 scan, BF16/FP8/SQ8/SQ4
+ * candidate scans with FP32 re-score, each followed by top-k selection.
+ *
+ * Plain C ABI: pointers and sizes only.  One handle = one device-resident
+ * index.  Results are row-major [B x k], ascending (distance, id); distances
+ * are squared L2 in fp32.  Calls on one handle are serialised internally.
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef ANNMI_H
+#define ANNMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* index modes (candidate-generation element type) */
+enum {
+    ANNMI_EXACT_F32 = 0, /* plain fp32 exact scan (src/linear_index.cpp)                   */
+    ANNMI_BF16 = 1,      /* bf16 MFMA candidates + fp32 re-score of R                       */
+    ANNMI_FP8 = 2,       /* e4m3 MFMA candidates + fp32 re-score of R                       */
+    ANNMI_SQ8 = 3,       /* per-dim SQ8 codes, int8 MFMA + fp32 re-score (gpu_filter.cu)    */
+    ANNMI_SQ4 = 4        /* per-dim SQ4 nibbles, int8 MFMA + fp32 re-score                  */
+};
+
+typedef struct annmi_params {
+    int32_t mode;           /* ANNMI_* */
+    int32_t rerank;         /* default over-fetch R for the quantized modes (0 -> 4k)         */
+    int32_t sq_variant;     /* SQ8 encoder: 0 = lsh_index_optimized.cpp:186-214 (std::round,
+                               constant dim offset 0), 1 = gpu_filter.cu:99-127 (lrintf,
+                               constant dim offset min)                                          */
+    int32_t base_on_device; /* 1: `base` is a device pointer, borrowed for the handle's life    */
+    int64_t max_batch;      /* queries per internal step (0 -> 4096)                           */
+    int64_t reserved[4];
+} annmi_params;
+
+/* Fills `p` with the defaults for `mode`. */
+void annmi_default_params(annmi_params* p, int32_t mode);
+
+/* Build an index over n x d row-major float32 `base`.
+ * Replaces: LinearIndex::fit        src/linear_index.cpp:73-88
+ *           gf_create               competitors/gpu_filter/src/gpu_filter.cu:79-140
+ * Returns NULL on error (annmi_last_error()).  A host `base` is copied; a
+ * device `base` (params->base_on_device) is borrowed when d % 16 == 0. */
+void* annmi_create(const float* base, int64_t n, int32_t d, const annmi_params* params);
+
+/* One query (host pointers).  Writes min(k, n) ids and distances.
+ * Replaces: LinearIndex::query      src/linear_index.cpp:90-121
+ *           gf_query                gpu_filter.cu:142-176 (rerank = r)
+ * rerank <= 0 uses the handle default.  Returns the count written or -1. */
+int32_t annmi_query(void* h, const float* q, int32_t k, int32_t rerank, int64_t* out_ids, float* out_dists);
+
+/* B queries, host pointers, outputs [B x k].  No reference counterpart: the
+ * batched form of annmi_query (the reference answers one query per call,
+ * orthogonal-competition/harness.py:159-174).  Returns min(k, n) or -1. */
+int32_t annmi_batch_query(void* h, const float* Q, int32_t B, int32_t k, int32_t rerank, int64_t* out_ids,
+                          float* out_dists);
+
+/* Same with device pointers (inputs must be ready when called); synchronous. */
+int32_t annmi_batch_query_device(void* h, const float* dQ, int32_t B, int32_t k, int32_t rerank, int64_t* d_ids,
+                                 float* d_dists);
+
+/* Full-precision distance evaluations so far.
+ * Replaces: LinearIndex::total_distances_count  src/linear_index.cpp:123 (N per query)
+ *           gf_ndist                             gpu_filter.cu:178 (R per query) */
+int64_t annmi_ndist(void* h);
+
+/* Per-query diagnostics of the last batch: rows re-scored and flags
+ * (bit0 candidate overflow, bit1 re-score truncated). Returns count or -1. */
+int32_t annmi_last_batch_stats(void* h, uint32_t* n_rescored, uint32_t* flags, int32_t max_q);
+
+/* Release the index.  NULL-safe.  Replaces gf_free gpu_filter.cu:180-190. */
+void annmi_free(void* h);
+
+/* Thread-local message of the last failure on this thread.
+ * Replaces gf_last_error gpu_filter.cu:77 (process-global there). */
+const char* annmi_last_error(void);
+
+/* Bytes of device memory held by the index (base copies, codes, norms). */
+int64_t annmi_index_bytes(void* h);
+
+/* Per-kernel timing with HIP events on the handle's own stream.  No
+ * reference counterpart (the reference's stage timers are
+ * lsh_index_optimized.cpp:535-548 / experiments/cuda/gpu_filter.cu:144-162).
+ * Copies the accumulated counters into out8 (when not NULL):
+ *   [0] scan launches  [1] scan ms  [2] scan algorithmic bytes  [3] scan
+ *   algorithmic flops (2*B*rows*d)  [4] threshold-select ms  [5] finalize ms
+ *   [6] query-prep ms  [7] steps
+ * then, when enable >= 0, switches timing on/off and zeroes the counters. */
+int32_t annmi_profile(void* h, int32_t enable, double* out8);
+
+/* ------------------------------------------------------------------
+ * Drop-in entry points with the exact signatures of the reference's GPU
+ * contestant, so competitors/gpu_filter/algorithm.py:6-15 binds this
+ * library unchanged: per-dim SQ8 (gpu_filter encoder), quantized scan,
+ * top-r, fp32 re-score of r on the GPU, top-k.
+ * ------------------------------------------------------------------ */
+void* gf_create(const float* train, long n, long dim);                      /* gpu_filter.cu:79  */
+int gf_query(void* handle, const float* q, int k, int r, long long* out_ids); /* gpu_filter.cu:142 */
+long long gf_ndist(void* handle);                                           /* gpu_filter.cu:178 */
+void gf_free(void* handle);                                                 /* gpu_filter.cu:180 */
+const char* gf_last_error(void);                                            /* gpu_filter.cu:77  */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ANNMI_H */
