@@ -632,6 +632,35 @@ const char* annmi_last_error(void) { return g_err; }
 
 int64_t annmi_index_bytes(void* h) { return h ? static_cast<Handle*>(h)->bytes : 0; }
 
+int64_t annmi_export(void* hv, int32_t what, void* out, int64_t bytes) {
+    Handle* h = static_cast<Handle*>(hv);
+    if (!h) {
+        set_err("null handle");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    const void* src = nullptr;
+    int64_t need = 0;
+    switch (what) {
+        case 0: src = h->Xs; need = h->n * h->xs_stride; break;
+        case 1: src = h->sq_scale; need = h->sq_scale ? (int64_t)h->kdim * 4 : 0; break;
+        case 2: src = h->sq_offset; need = h->sq_offset ? (int64_t)h->kdim * 4 : 0; break;
+        case 3: src = h->xn; need = h->n * 4; break;
+        case 4: {
+            // scalar parameters: kdim, row stride (bytes), fp8 alpha, max row norm
+            if (out && bytes >= 32) {
+                double v[4] = {(double)h->kdim, (double)h->xs_stride, (double)h->alpha_x, (double)h->maxnorm};
+                std::memcpy(out, v, 32);
+            }
+            return 32;
+        }
+        default: set_err("unknown export %d", what); return -1;
+    }
+    if (!out || bytes < need) return need;
+    HIP_OR(hipMemcpy(out, src, (size_t)need, hipMemcpyDeviceToHost), -1);
+    return need;
+}
+
 int32_t annmi_profile(void* hv, int32_t enable, double* out8) {
     Handle* h = static_cast<Handle*>(hv);
     if (!h) {

@@ -35,7 +35,7 @@ struct PolF32 {
     static constexpr int kDimsPerLoad = 4;     // per lane per 16B load
     static constexpr int kChunk = 16;          // dims per wave-wide load step (4 lane groups)
     static constexpr int kElemBytes = 4;
-    typedef float Acc;
+    typedef v4f Acc;
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         f32x4 af = __builtin_bit_cast(f32x4, a);
         f32x4 bf = __builtin_bit_cast(f32x4, b);
@@ -50,6 +50,7 @@ struct PolBF16 {
     static constexpr int kDimsPerLoad = 8;
     static constexpr int kChunk = 32;
     static constexpr int kElemBytes = 2;
+    typedef v4f Acc;
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
                                                       __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
@@ -60,6 +61,7 @@ struct PolFP8 {
     static constexpr int kDimsPerLoad = 16;
     static constexpr int kChunk = 64;
     static constexpr int kElemBytes = 1;
+    typedef v4f Acc;
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         long a0 = ((long)(uint32_t)a[1] << 32) | (uint32_t)a[0];
         long a1 = ((long)(uint32_t)a[3] << 32) | (uint32_t)a[2];
@@ -70,19 +72,17 @@ struct PolFP8 {
     }
 };
 
-// int8 MFMA with an int32 accumulator, kept in a float-typed v4f register
-// image between calls (bit casts only, no conversion).
+// int8 MFMA with an int32 accumulator.
 struct PolSQ8 {
     static constexpr int kDimsPerLoad = 16;
     static constexpr int kChunk = 64;
     static constexpr int kElemBytes = 1;
-    __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
+    typedef i32x4 Acc;
+    __device__ static inline void mma(i32x4& acc, const i32x4& a, const i32x4& b) {
         // codes are uint8 in [0,255]; the MFMA reads signed int8, so flip the
         // top bit: c - 128.  The query side adds 128*sum(w) back.
         i32x4 bs = b ^ (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
-        i32x4 ai = __builtin_bit_cast(i32x4, acc);
-        ai = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bs, ai, 0, 0, 0);
-        acc = __builtin_bit_cast(v4f, ai);
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bs, acc, 0, 0, 0);
     }
 };
 
@@ -93,25 +93,12 @@ struct PolSQ4 {
     static constexpr int kDimsPerLoad = 16;
     static constexpr int kChunk = 64;
     static constexpr int kElemBytes = 0;  // packed: 2 dims per byte
-    __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
-        i32x4 ai = __builtin_bit_cast(i32x4, acc);
-        ai = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, ai, 0, 0, 0);
-        acc = __builtin_bit_cast(v4f, ai);
+    typedef i32x4 Acc;
+    __device__ static inline void mma(i32x4& acc, const i32x4& a, const i32x4& b) {
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
     }
 };
 
-template <class P>
-struct IsInt {
-    static constexpr bool v = false;
-};
-template <>
-struct IsInt<PolSQ8> {
-    static constexpr bool v = true;
-};
-template <>
-struct IsInt<PolSQ4> {
-    static constexpr bool v = true;
-};
 
 // Row fragment load: 16 bytes of row `row` covering dims [dim0, dim0 + kDimsPerLoad).
 template <class P>
@@ -186,9 +173,9 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
         const int64_t row = a.row_begin + t * 16 + n;
         const bool row_ok = row < a.row_end;
         const int64_t lrow = row_ok ? row : a.row_end - 1;
-        v4f acc[QW];
+        typename P::Acc acc[QW];
 #pragma unroll
-        for (int w = 0; w < QW; ++w) acc[w] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int w = 0; w < QW; ++w) acc[w] = typename P::Acc{};
 
         constexpr int U = 8;
         i32x4 buf[U];
@@ -224,12 +211,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
         for (int w = 0; w < QW; ++w) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float dot;
-                if constexpr (IsInt<P>::v) {
-                    dot = (float)__builtin_bit_cast(int, acc[w][i]);
-                } else {
-                    dot = acc[w][i];
-                }
+                const float dot = (float)acc[w][i];
                 // dist = xn + qn - 2 * qscale * dot  (qscale folds the query/base scaling)
                 float dist = xn + qn[w][i] - 2.0f * qs[w][i] * dot;
                 bool pass = row_ok && dist <= thr[w][i];
@@ -338,11 +320,11 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
             rok[r] = row < r_end;
             lrow[r] = rok[r] ? row : r_end - 1;
         }
-        v4f acc[QS][RS];
+        typename P::Acc acc[QS][RS];
 #pragma unroll
         for (int q = 0; q < QS; ++q)
 #pragma unroll
-            for (int r = 0; r < RS; ++r) acc[q][r] = (v4f){0.f, 0.f, 0.f, 0.f};
+            for (int r = 0; r < RS; ++r) acc[q][r] = typename P::Acc{};
         i32x4 bc[RS], bn[RS];
 #pragma unroll
         for (int r = 0; r < RS; ++r) bc[r] = load_row_frag<P>(X, a.xstride, lrow[r], lane_dim);
@@ -366,6 +348,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
 #pragma unroll
         for (int r = 0; r < RS; ++r) xn[r] = a.xn[lrow[r]];
         bool any = false;
+        v4f dist4[QS][RS];
 #pragma unroll
         for (int q = 0; q < QS; ++q)
 #pragma unroll
@@ -374,9 +357,8 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                 const float th = thr_s[m], qv = qn_s[m], sc = 2.0f * qsc_s[m];
 #pragma unroll
                 for (int r = 0; r < RS; ++r) {
-                    float dot = IsInt<P>::v ? (float)__builtin_bit_cast(int, acc[q][r][i]) : acc[q][r][i];
-                    float dist = xn[r] + qv - sc * dot;
-                    acc[q][r][i] = dist;
+                    float dist = xn[r] + qv - sc * (float)acc[q][r][i];
+                    dist4[q][r][i] = dist;
                     any |= rok[r] && dist <= th;
                 }
             }
@@ -390,7 +372,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                 const int qi = q0 + m;
 #pragma unroll
                 for (int r = 0; r < RS; ++r) {
-                    const float dist = acc[q][r][i];
+                    const float dist = dist4[q][r][i];
                     const bool pass = rok[r] && dist <= th;
                     uint64_t mb = __ballot(pass);
                     if (mb == 0) continue;

@@ -95,6 +95,28 @@ class Index:
                                              f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), B)
         return n[:got], f[:got]
 
+    def export(self, what: str):
+        """Host copy of index state: 'rows' (scan rows as uint8 [n, stride]),
+        'scale', 'offset' (SQ per-dim, float32), 'norms' (float32 [n]),
+        'meta' (dict)."""
+        codes = {"rows": 0, "scale": 1, "offset": 2, "norms": 3, "meta": 4}
+        w = codes[what]
+        need = self._L.annmi_export(self._h, w, None, 0)
+        if need < 0:
+            raise RuntimeError(_lib.last_error())
+        if w == 4:
+            buf = np.zeros(4, dtype=np.float64)
+            self._L.annmi_export(self._h, 4, buf.ctypes.data_as(ctypes.c_void_p), 32)
+            return {"kdim": int(buf[0]), "row_stride": int(buf[1]), "fp8_alpha": float(buf[2]),
+                    "max_norm": float(buf[3])}
+        buf = np.empty(int(need), dtype=np.uint8)
+        if need and self._L.annmi_export(self._h, w, buf.ctypes.data_as(ctypes.c_void_p), int(need)) < 0:
+            raise RuntimeError(_lib.last_error())
+        if w in (1, 2, 3):
+            return buf.view(np.float32)
+        stride = self.export("meta")["row_stride"]
+        return buf.reshape(self.n, stride)
+
     def profile(self, enable: int = -1) -> dict:
         """Per-kernel event timing counters (see annmi_profile); enable=1 starts
         a fresh measurement, 0 stops, -1 only reads."""

@@ -82,6 +82,18 @@ const char* annmi_last_error(void);
 /* Bytes of device memory held by the index (base copies, codes, norms). */
 int64_t annmi_index_bytes(void* h);
 
+/* Copy index state to host memory (bytes >= the returned size), e.g. to
+ * check encoders or persist an index.  No reference counterpart (the
+ * reference keeps its codes private and has no persistence, README.md:12-13).
+ *   what 0: scan rows (n x row-stride bytes: fp32 / bf16 / e4m3 / SQ8 codes /
+ *           SQ4 nibbles in the submission/src/hnsw.cpp:170-182 layout)
+ *        1: SQ per-dim scale (kdim floats)   2: SQ per-dim offset (kdim floats)
+ *        3: per-row norm term (n floats)     4: 4 doubles {kdim, row stride
+ *           bytes, fp8 scale, max row norm}
+ * With out == NULL or bytes too small, returns the size without copying;
+ * -1 on error. */
+int64_t annmi_export(void* h, int32_t what, void* out, int64_t bytes);
+
 /* Per-kernel timing with HIP events on the handle's own stream.  No
  * reference counterpart (the reference's stage timers are
  * lsh_index_optimized.cpp:535-548 / experiments/cuda/gpu_filter.cu:144-162).
