@@ -25,7 +25,7 @@ MODES = {"exact": ANNMI_EXACT_F32, "fp32": ANNMI_EXACT_F32, "bf16": ANNMI_BF16, 
 EXPORTS = [
     "annmi_default_params", "annmi_create", "annmi_query", "annmi_batch_query", "annmi_batch_query_device",
     "annmi_ndist", "annmi_last_batch_stats", "annmi_free", "annmi_last_error", "annmi_index_bytes",
-    "annmi_profile", "annmi_export",
+    "annmi_profile", "annmi_profile_kernels", "annmi_export",
     "gf_create", "gf_query", "gf_ndist", "gf_free", "gf_last_error",
 ]
 
@@ -86,6 +86,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.annmi_last_error.argtypes = []
     L.annmi_index_bytes.restype = i64
     L.annmi_index_bytes.argtypes = [vp]
+    L.annmi_profile_kernels.restype = i32
+    L.annmi_profile_kernels.argtypes = [vp, ctypes.c_char_p, i32]
     L.annmi_export.restype = i64
     L.annmi_export.argtypes = [vp, i32, vp, i64]
     L.annmi_profile.restype = i32

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -146,6 +147,10 @@ struct Handle {
     bool prof = false;
     double prof_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // scan launches, scan ms, scan bytes, scan flops, select ms, finalize ms, prep ms, steps
     std::vector<hipEvent_t> ev_pool;
+    struct KStat {
+        double launches = 0, ms = 0, bytes = 0, flops = 0;
+    };
+    std::map<std::string, KStat> kstats;
     std::atomic<int64_t> ndist{0};
     std::mutex mu;
 };
@@ -244,6 +249,7 @@ bool run_step(Handle* h, int B, int k, int R, int64_t* ids_out, float* d_out) {
     // event slots: [0] step start, then per scan launch 2, select 2 each, finalize 2
     std::vector<std::pair<int, int>> scan_ev, sel_ev;  // (start idx, end idx)
     std::vector<double> scan_rows;
+    std::vector<std::string> scan_names;
     int ev_used = 0;
     auto ev = [&](void) -> int {
         if (!h->prof) return -1;
@@ -290,10 +296,12 @@ bool run_step(Handle* h, int B, int k, int R, int64_t* ids_out, float* d_out) {
         a.B = B;
         a.cand = cb;
         int e0 = ev();
-        HIP_OR(launch_scan(a, st), false);
+        const char* vname = nullptr;
+        HIP_OR(launch_scan(a, st, &vname), false);
         int e1 = ev();
         scan_ev.push_back({e0, e1});
         scan_rows.push_back((double)(a.row_end - a.row_begin));
+        scan_names.push_back(vname ? vname : "");
     }
     FinalArgs fa{};
     fa.cand = cb;
@@ -327,10 +335,18 @@ bool run_step(Handle* h, int B, int k, int R, int64_t* ids_out, float* d_out) {
         };
         const double row_bytes = (double)h->xs_stride;
         for (size_t j = 0; j < scan_ev.size(); ++j) {
+            const double t = ms(scan_ev[j].first, scan_ev[j].second);
+            const double by = scan_rows[j] * row_bytes + (double)B * scan_q_bytes(h->kind, h->kdim);
+            const double fl = 2.0 * (double)B * scan_rows[j] * (double)h->d;
             h->prof_acc[0] += 1;
-            h->prof_acc[1] += ms(scan_ev[j].first, scan_ev[j].second);
-            h->prof_acc[2] += scan_rows[j] * row_bytes + (double)B * scan_q_bytes(h->kind, h->kdim);
-            h->prof_acc[3] += 2.0 * (double)B * scan_rows[j] * (double)h->d;
+            h->prof_acc[1] += t;
+            h->prof_acc[2] += by;
+            h->prof_acc[3] += fl;
+            Handle::KStat& k = h->kstats[scan_names[j]];
+            k.launches += 1;
+            k.ms += t;
+            k.bytes += by;
+            k.flops += fl;
         }
         for (auto& e : sel_ev) h->prof_acc[4] += ms(e.first, e.second);
         h->prof_acc[5] += ms(ev_f0, ev_f1);
@@ -673,8 +689,30 @@ int32_t annmi_profile(void* hv, int32_t enable, double* out8) {
     if (enable >= 0) {
         h->prof = enable != 0;
         for (int i = 0; i < 8; ++i) h->prof_acc[i] = 0;
+        h->kstats.clear();
     }
     return 8;
+}
+
+int32_t annmi_profile_kernels(void* hv, char* buf, int32_t len) {
+    Handle* h = static_cast<Handle*>(hv);
+    if (!h) {
+        set_err("null handle");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    std::string out = "[";
+    char line[256];
+    bool first = true;
+    for (auto& kv : h->kstats) {
+        snprintf(line, sizeof(line), "%s{\"kernel\":\"%s\",\"launches\":%.0f,\"ms\":%.6f,\"bytes\":%.6g,\"flops\":%.6g}",
+                 first ? "" : ",", kv.first.c_str(), kv.second.launches, kv.second.ms, kv.second.bytes, kv.second.flops);
+        out += line;
+        first = false;
+    }
+    out += "]";
+    if (buf && len > 0) snprintf(buf, (size_t)len, "%s", out.c_str());
+    return (int32_t)out.size() + 1;
 }
 
 // ------------------------------------------------ gpu_filter drop-in ABI --

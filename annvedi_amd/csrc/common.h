@@ -73,7 +73,9 @@ struct ScanArgs {
     int B;
     CandBuf cand;
 };
-hipError_t launch_scan(const ScanArgs& a, hipStream_t st);
+// Launches the scan; *variant (if not null) receives a static name of the
+// kernel instantiation launched (for per-kernel timing).
+hipError_t launch_scan(const ScanArgs& a, hipStream_t st, const char** variant = nullptr);
 
 
 struct SelectArgs {

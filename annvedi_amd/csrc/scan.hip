@@ -26,6 +26,8 @@
 // the hardware's internal k order.
 #include "common.h"
 
+#include <cstdio>
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -235,8 +237,21 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
     }
 }
 
+template <class P>
+const char* pol_name();
+template <> const char* pol_name<PolF32>() { return "f32"; }
+template <> const char* pol_name<PolBF16>() { return "bf16"; }
+template <> const char* pol_name<PolFP8>() { return "fp8"; }
+template <> const char* pol_name<PolSQ8>() { return "sq8"; }
+template <> const char* pol_name<PolSQ4>() { return "sq4"; }
+
+thread_local const char* t_variant = "";
+
 template <class P, int QW>
 hipError_t launch_t(const ScanArgs& a, hipStream_t st) {
+    static char name[64] = "";
+    if (!name[0]) snprintf(name, sizeof(name), "scan_kernel<%s,QW=%d>", pol_name<P>(), QW);
+    t_variant = name;
     const int qtile = 16 * QW;
     const int qtiles = (a.B + qtile - 1) / qtile;
     const int64_t rows = a.row_end - a.row_begin;
@@ -395,6 +410,9 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
 
 template <class P, int QS, int RS, int NW>
 hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
+    static char name[64] = "";
+    if (!name[0]) snprintf(name, sizeof(name), "scan_wide_kernel<%s,QS=%d,RS=%d,NW=%d>", pol_name<P>(), QS, RS, NW);
+    t_variant = name;
     static int ncu = 0;
     if (ncu == 0) {
         int dev = 0;
@@ -449,7 +467,16 @@ hipError_t launch_p(const ScanArgs& a, hipStream_t st) {
 
 int g_wide_min_batch = 128;
 
-hipError_t launch_scan(const ScanArgs& a, hipStream_t st) {
+hipError_t launch_scan_impl(const ScanArgs& a, hipStream_t st);
+
+hipError_t launch_scan(const ScanArgs& a, hipStream_t st, const char** variant) {
+    t_variant = "";
+    hipError_t e = launch_scan_impl(a, st);
+    if (variant) *variant = t_variant;
+    return e;
+}
+
+hipError_t launch_scan_impl(const ScanArgs& a, hipStream_t st) {
     if (a.row_end <= a.row_begin || a.B <= 0) return hipSuccess;
     if (a.B >= g_wide_min_batch && (a.row_end - a.row_begin) >= 65536) {
         switch (a.kind) {

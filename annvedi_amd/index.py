@@ -126,6 +126,17 @@ class Index:
         keys = ["scan_launches", "scan_ms", "scan_bytes", "scan_flops", "select_ms", "finalize_ms", "prep_ms", "steps"]
         return dict(zip(keys, list(out)))
 
+    def profile_kernels(self) -> list:
+        """Per scan-kernel instantiation: launches, ms, algorithmic bytes and flops."""
+        import json
+
+        need = self._L.annmi_profile_kernels(self._h, None, 0)
+        if need < 0:
+            raise RuntimeError(_lib.last_error())
+        buf = ctypes.create_string_buffer(int(need))
+        self._L.annmi_profile_kernels(self._h, buf, int(need))
+        return json.loads(buf.value.decode())
+
     @property
     def n_distances(self) -> int:
         return int(self._L.annmi_ndist(self._h))

@@ -104,6 +104,11 @@ int64_t annmi_export(void* h, int32_t what, void* out, int64_t bytes);
  * then, when enable >= 0, switches timing on/off and zeroes the counters. */
 int32_t annmi_profile(void* h, int32_t enable, double* out8);
 
+/* The same counters split by scan kernel instantiation, as a JSON array of
+ * {kernel, launches, ms, bytes, flops} written to buf (len bytes).  Returns
+ * the length needed including the terminator, -1 on error. */
+int32_t annmi_profile_kernels(void* h, char* buf, int32_t len);
+
 /* ------------------------------------------------------------------
  * Drop-in entry points with the exact signatures of the reference's GPU
  * contestant, so competitors/gpu_filter/algorithm.py:6-15 binds this
