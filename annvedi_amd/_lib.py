@@ -17,13 +17,15 @@ ANNMI_BF16 = 1
 ANNMI_FP8 = 2
 ANNMI_SQ8 = 3
 ANNMI_SQ4 = 4
+ANNMI_IVF_SQ8 = 5
 
 MODES = {"exact": ANNMI_EXACT_F32, "fp32": ANNMI_EXACT_F32, "bf16": ANNMI_BF16, "fp8": ANNMI_FP8,
-         "sq8": ANNMI_SQ8, "sq4": ANNMI_SQ4}
+         "sq8": ANNMI_SQ8, "sq4": ANNMI_SQ4, "ivf_sq8": ANNMI_IVF_SQ8, "ivf": ANNMI_IVF_SQ8}
 
 # every function declared in include/annmi.h
 EXPORTS = [
     "annmi_default_params", "annmi_create", "annmi_query", "annmi_batch_query", "annmi_batch_query_device",
+    "annmi_batch_query_ex",
     "annmi_ndist", "annmi_last_batch_stats", "annmi_free", "annmi_last_error", "annmi_index_bytes",
     "annmi_profile", "annmi_profile_kernels", "annmi_export",
     "gf_create", "gf_query", "gf_ndist", "gf_free", "gf_last_error",
@@ -37,7 +39,11 @@ class Params(ctypes.Structure):
         ("sq_variant", ctypes.c_int32),
         ("base_on_device", ctypes.c_int32),
         ("max_batch", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 4),
+        ("nlist", ctypes.c_int32),
+        ("nprobe", ctypes.c_int32),
+        ("kmeans_iters", ctypes.c_int32),
+        ("kmeans_seed", ctypes.c_int32),
+        ("reserved", ctypes.c_int64 * 2),
     ]
 
 
@@ -74,6 +80,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.annmi_query.argtypes = [vp, fp, i32, i32, i64p, fp]
     L.annmi_batch_query.restype = i32
     L.annmi_batch_query.argtypes = [vp, fp, i32, i32, i32, i64p, fp]
+    L.annmi_batch_query_ex.restype = i32
+    L.annmi_batch_query_ex.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp, i32]
     L.annmi_batch_query_device.restype = i32
     L.annmi_batch_query_device.argtypes = [vp, vp, i32, i32, i32, vp, vp]
     L.annmi_ndist.restype = i64

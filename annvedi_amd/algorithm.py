@@ -7,9 +7,12 @@ competitors/gpu_filter/algorithm.py:30-59): ``fit(train, **index_params)``,
 ``batch_query(Q, k, **query_params) -> (ids int64[B, k], dists float32[B, k])``.
 
 Selected in scenarios.yaml with ``backend: mi355x`` and ``mode`` in
-{exact, bf16, fp8, sq8, sq4}; ``rerank`` (alias ``r``, as in
-competitors/gpu_filter/scenarios.yaml) is the over-fetch for the quantized
-modes.  ``ds_size`` truncates the base as algorithm.py:19-21 does.
+{exact, bf16, fp8, sq8, sq4, ivf}; ``rerank`` (aliases ``r`` as in
+competitors/gpu_filter/scenarios.yaml, ``refine_r`` as in the ivf_lsh fork)
+is the over-fetch for the quantized modes; IVF takes ``nlist`` (alias
+``n_clusters``) at fit and ``nprobe`` (alias ``n_probe_clusters``) at query,
+as competitors/ivf_lsh/algorithm.py passes them.  ``ds_size`` truncates the
+base as algorithm.py:19-21 does.
 
 ``get_n_distances`` counts full-precision distance evaluations: N per query
 in exact mode (src/linear_index.cpp:108), the re-scored rows per query in the
@@ -29,6 +32,7 @@ class Algorithm:
         self._backend = "mi355x"
         self._index = None
         self._rerank = 0
+        self._nprobe = 0
 
     def fit(self, train: np.ndarray, **index_params) -> None:
         ds_size = index_params.pop("ds_size", None)
@@ -39,12 +43,19 @@ class Algorithm:
             raise ValueError(f"backend {self._backend!r} is not served by this plugin ({BACKENDS})")
         # 'linear' keeps the reference's exact-scan meaning
         mode = str(index_params.pop("mode", "exact")).lower()
-        self._rerank = int(index_params.pop("rerank", index_params.pop("r", 0)) or 0)
+        self._rerank = int(index_params.pop("rerank", index_params.pop("r", index_params.pop("refine_r", 0))) or 0)
         sq_variant = int(index_params.pop("sq_variant", 0))
         max_batch = int(index_params.pop("max_batch", 4096))
+        # IVF (competitors/ivf_lsh: fit(n_clusters) :128, query(n_probe_clusters, refine_r) :304)
+        nlist = int(index_params.pop("nlist", index_params.pop("n_clusters", 0)) or 0)
+        self._nprobe = int(index_params.pop("nprobe", index_params.pop("n_probe_clusters", 0)) or 0)
+        kmeans_iters = int(index_params.pop("kmeans_iters", 0) or 0)
+        for unused in ("n_tables", "n_bits"):   # LSH buckets of the ivf_lsh fork: a full-list scan here
+            index_params.pop(unused, None)
         if index_params:
             raise ValueError(f"unknown index_params: {sorted(index_params)}")
-        self._index = Index(train, mode=mode, rerank=self._rerank, sq_variant=sq_variant, max_batch=max_batch)
+        self._index = Index(train, mode=mode, rerank=self._rerank, sq_variant=sq_variant, max_batch=max_batch,
+                            nlist=nlist, nprobe=self._nprobe, kmeans_iters=kmeans_iters)
 
     def _need_index(self):
         if self._index is None:
@@ -55,19 +66,24 @@ class Algorithm:
         q = np.asarray(query, dtype=np.float32)
         if q.ndim != 1 or q.shape[0] != self._index.d:
             raise RuntimeError(f"query must be a 1-D vector of length {self._index.d}")
-        r = int(query_params.get("rerank", query_params.get("r", self._rerank)) or 0)
-        ids, _ = self._index.search(q[None, :], int(k), rerank=r)
+        r, npb = self._qp(query_params)
+        ids, _ = self._index.search(q[None, :], int(k), rerank=r, nprobe=npb)
         return ids[0]
+
+    def _qp(self, qp):
+        r = int(qp.get("rerank", qp.get("r", qp.get("refine_r", self._rerank))) or 0)
+        npb = int(qp.get("nprobe", qp.get("n_probe_clusters", self._nprobe)) or 0)
+        return max(r, 0), max(npb, 0)
 
     def batch_query(self, queries, k: int, **query_params):
         self._need_index()
-        r = int(query_params.get("rerank", query_params.get("r", self._rerank)) or 0)
+        r, npb = self._qp(query_params)
         if _is_device(queries):
-            return self._index.search_device(queries, int(k), rerank=r)
+            return self._index.search_device(queries, int(k), rerank=r, nprobe=npb)
         Q = np.asarray(queries, dtype=np.float32)
         if Q.ndim != 2 or Q.shape[1] != self._index.d:
             raise RuntimeError(f"queries must have shape (B, {self._index.d})")
-        return self._index.search(Q, int(k), rerank=r)
+        return self._index.search(Q, int(k), rerank=r, nprobe=npb)
 
     def get_n_distances(self) -> int:
         if self._index is None:

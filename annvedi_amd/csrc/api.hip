@@ -28,10 +28,14 @@
 
 #include "../../include/annmi.h"
 #include "common.h"
+#include "ivf.h"
+
+#include <numeric>
+#include <random>
 
 hipError_t launch_encode(int kind, const float* src, int64_t sstride, int64_t n, int d, int kdim, void* dst, float* xn,
                          uint32_t* maxkey, float alpha, const float* scale, const float* offset, int variant,
-                         hipStream_t st);
+                         hipStream_t st, const uint32_t* gather = nullptr);
 hipError_t launch_dim_minmax(const float* src, int64_t sstride, int64_t n, int d, uint32_t* mn, uint32_t* mx,
                              hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
@@ -76,6 +80,7 @@ int scan_kind_of(int mode) {
         case ANNMI_FP8: return SCAN_FP8;
         case ANNMI_SQ8: return SCAN_SQ8;
         case ANNMI_SQ4: return SCAN_SQ4;
+        case ANNMI_IVF_SQ8: return SCAN_SQ8;
     }
     return -1;
 }
@@ -153,6 +158,23 @@ struct Handle {
     std::map<std::string, KStat> kstats;
     std::atomic<int64_t> ndist{0};
     std::mutex mu;
+    // IVF (mode ANNMI_IVF_SQ8): lists are contiguous runs of the list-ordered
+    // scan rows Xs; perm maps list position -> original row id.
+    bool ivf = false;
+    int nlist = 0;
+    uint32_t* perm = nullptr;
+    std::vector<int64_t> list_off;  // host, nlist + 1
+    float* centroids = nullptr;     // [nlist x kdim32]
+    struct Handle* cent = nullptr;  // exact index over the centroids (probe step)
+    int64_t* d_probe = nullptr;     // [ws_B x nprobe_cap]
+    float* d_probe_d = nullptr;
+    int probe_cap = 0;
+    IvfItem* d_items = nullptr;
+    int items_cap = 0;
+    uint32_t* d_gq = nullptr;
+    int64_t gq_cap = 0;
+    uint32_t* d_next = nullptr;
+    std::vector<int64_t> last_scanned;
 };
 
 void free_ws(Handle* h) {
@@ -190,6 +212,14 @@ void free_handle(Handle* h) {
     dfree(h->sq_scale);
     dfree(h->sq_offset);
     dfree(h->scratch_keys);
+    dfree(h->perm);
+    dfree(h->centroids);
+    dfree(h->d_probe);
+    dfree(h->d_probe_d);
+    dfree(h->d_items);
+    dfree(h->d_gq);
+    dfree(h->d_next);
+    if (h->cent) free_handle(h->cent);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
@@ -235,23 +265,17 @@ std::vector<int64_t> range_bounds(int64_t n, int rsel, int cap) {
     return b;
 }
 
-// One step over <= ws_B queries already in h->dQ32; results in d_ids/d_dist.
-bool run_step(Handle* h, int B, int k, int R, int64_t* ids_out, float* d_out) {
-    const bool exact = h->kind == SCAN_F32;
-    const int kk = (int)std::min<int64_t>(k, h->n);
-    int rsel = exact ? kk : std::max(kk, R);
-    rsel = (int)std::min<int64_t>(rsel, h->n);
-    if (!exact && rsel > 2048) {
-        set_err("rerank R=%d exceeds the 2048-row re-score limit", rsel);
-        return false;
-    }
-    hipStream_t st = h->st;
-    // event slots: [0] step start, then per scan launch 2, select 2 each, finalize 2
-    std::vector<std::pair<int, int>> scan_ev, sel_ev;  // (start idx, end idx)
-    std::vector<double> scan_rows;
-    std::vector<std::string> scan_names;
-    int ev_used = 0;
-    auto ev = [&](void) -> int {
+// Event bookkeeping shared by the flat and IVF steps.
+struct Profiler {
+    Handle* h;
+    int& ev_used;
+    std::vector<std::pair<int, int>>* scan_ev;
+    std::vector<std::pair<int, int>>* sel_ev;
+    std::vector<double>* scan_rows;
+    std::vector<std::string>* scan_names;
+    int ev_prep0, ev_prep1;
+    std::vector<double> bytes_override, flops_override;  // per scan launch, when set
+    int ev() {
         if (!h->prof) return -1;
         if (ev_used == (int)h->ev_pool.size()) {
             hipEvent_t e;
@@ -259,50 +283,15 @@ bool run_step(Handle* h, int B, int k, int R, int64_t* ids_out, float* d_out) {
             h->ev_pool.push_back(e);
         }
         int i = ev_used++;
-        (void)hipEventRecord(h->ev_pool[i], st);
+        (void)hipEventRecord(h->ev_pool[i], h->st);
         return i;
-    };
-    int ev_prep0 = ev();
-    HIP_OR(launch_prep_queries(h->kind, h->dQ32, B, h->d, h->kdim, h->dQs, h->qn, h->qscale, h->slack, h->maxnorm,
-                               h->alpha_x, h->sq_scale, h->sq_offset, st),
-           false);
-    HIP_OR(launch_fill_u32(h->cnt, 0u, B, st), false);
-    HIP_OR(launch_fill_u32(h->ovf, 0u, B, st), false);
-    HIP_OR(launch_fill_f32(h->thr, __builtin_inff(), B, st), false);
-    CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
-    int ev_prep1 = ev();
-    std::vector<int64_t> bounds = range_bounds(h->n, rsel, h->cap);
-    for (size_t j = 0; j + 1 < bounds.size(); ++j) {
-        if (j > 0) {
-            SelectArgs sa{cb, B, rsel, h->slack, h->thr, h->ovf};
-            int e0 = ev();
-            HIP_OR(launch_select_threshold(sa, st), false);
-            int e1 = ev();
-            sel_ev.push_back({e0, e1});
-        }
-        ScanArgs a{};
-        a.kind = h->kind;
-        a.X = h->Xs;
-        a.xstride = h->xs_stride;
-        a.xn = h->xn;
-        a.row_begin = bounds[j];
-        a.row_end = bounds[j + 1];
-        a.d = h->d;
-        a.kdim = h->kdim;
-        a.Q = h->dQs;
-        a.qn = h->qn;
-        a.qscale = exact ? nullptr : h->qscale;
-        a.thr = h->thr;
-        a.B = B;
-        a.cand = cb;
-        int e0 = ev();
-        const char* vname = nullptr;
-        HIP_OR(launch_scan(a, st, &vname), false);
-        int e1 = ev();
-        scan_ev.push_back({e0, e1});
-        scan_rows.push_back((double)(a.row_end - a.row_begin));
-        scan_names.push_back(vname ? vname : "");
     }
+};
+
+// finalize + flag read-back + accounting, common to both step kinds.
+bool finish_step(Handle* h, int B, int k, int kk, int rsel, int64_t* ids_out, float* d_out, const CandBuf& cb,
+                 bool exact, Profiler& pr) {
+    hipStream_t st = h->st;
     FinalArgs fa{};
     fa.cand = cb;
     fa.B = B;
@@ -320,14 +309,19 @@ bool run_step(Handle* h, int B, int k, int R, int64_t* ids_out, float* d_out) {
     // fp32 re-score error bound (relative, both neighbours), doubled
     const double u = 5.9604644775390625e-8;
     fa.eps_rel = 4.0 * ((h->d + 15) / 16 + 28) * u;
-    int ev_f0 = ev();
+    int ev_f0 = pr.ev();
     HIP_OR(launch_finalize(fa, st), false);
-    int ev_f1 = ev();
+    int ev_f1 = pr.ev();
     std::vector<uint32_t> flags(B), nrr(B);
     HIP_OR(hipMemcpyAsync(flags.data(), h->ovf, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
     HIP_OR(hipMemcpyAsync(nrr.data(), h->nrr, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
     HIP_OR(hipStreamSynchronize(st), false);
     if (h->prof && ev_f1 >= 0) {
+        auto& scan_ev = *pr.scan_ev;
+        auto& sel_ev = *pr.sel_ev;
+        auto& scan_rows = *pr.scan_rows;
+        auto& scan_names = *pr.scan_names;
+        const int ev_prep0 = pr.ev_prep0, ev_prep1 = pr.ev_prep1;
         auto ms = [&](int a0, int a1) -> double {
             float t = 0.f;
             if (a0 < 0 || a1 < 0 || hipEventElapsedTime(&t, h->ev_pool[a0], h->ev_pool[a1]) != hipSuccess) return 0.0;
@@ -336,8 +330,11 @@ bool run_step(Handle* h, int B, int k, int R, int64_t* ids_out, float* d_out) {
         const double row_bytes = (double)h->xs_stride;
         for (size_t j = 0; j < scan_ev.size(); ++j) {
             const double t = ms(scan_ev[j].first, scan_ev[j].second);
-            const double by = scan_rows[j] * row_bytes + (double)B * scan_q_bytes(h->kind, h->kdim);
-            const double fl = 2.0 * (double)B * scan_rows[j] * (double)h->d;
+            const double by = j < pr.bytes_override.size()
+                                  ? pr.bytes_override[j]
+                                  : scan_rows[j] * row_bytes + (double)B * scan_q_bytes(h->kind, h->kdim);
+            const double fl = j < pr.flops_override.size() ? pr.flops_override[j]
+                                                           : 2.0 * (double)B * scan_rows[j] * (double)h->d;
             h->prof_acc[0] += 1;
             h->prof_acc[1] += t;
             h->prof_acc[2] += by;
@@ -367,10 +364,80 @@ bool run_step(Handle* h, int B, int k, int R, int64_t* ids_out, float* d_out) {
     return true;
 }
 
+
+bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, float* d_out);
+int32_t batch_impl(Handle* h, const float* Q, int32_t B, int32_t k, int32_t R, int64_t* ids, float* dists,
+                   bool device, int32_t nprobe);
+Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params* params, int64_t src_stride);
+
+// One step over <= ws_B queries already in h->dQ32; results in d_ids/d_dist.
+bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, float* d_out) {
+    if (h->ivf) return run_step_ivf(h, B, k, R, nprobe, ids_out, d_out);
+    const bool exact = h->kind == SCAN_F32;
+    const int kk = (int)std::min<int64_t>(k, h->n);
+    int rsel = exact ? kk : std::max(kk, R);
+    rsel = (int)std::min<int64_t>(rsel, h->n);
+    if (!exact && rsel > 2048) {
+        set_err("rerank R=%d exceeds the 2048-row re-score limit", rsel);
+        return false;
+    }
+    hipStream_t st = h->st;
+    std::vector<std::pair<int, int>> scan_ev, sel_ev;  // (start, end) event indices
+    std::vector<double> scan_rows;
+    std::vector<std::string> scan_names;
+    int ev_used = 0;
+    Profiler pr{h, ev_used, &scan_ev, &sel_ev, &scan_rows, &scan_names, -1, -1};
+    pr.ev_prep0 = pr.ev();
+    HIP_OR(launch_prep_queries(h->kind, h->dQ32, B, h->d, h->kdim, h->dQs, h->qn, h->qscale, h->slack, h->maxnorm,
+                               h->alpha_x, h->sq_scale, h->sq_offset, st),
+           false);
+    HIP_OR(launch_fill_u32(h->cnt, 0u, B, st), false);
+    HIP_OR(launch_fill_u32(h->ovf, 0u, B, st), false);
+    HIP_OR(launch_fill_f32(h->thr, __builtin_inff(), B, st), false);
+    CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
+    pr.ev_prep1 = pr.ev();
+    std::vector<int64_t> bounds = range_bounds(h->n, rsel, h->cap);
+    for (size_t j = 0; j + 1 < bounds.size(); ++j) {
+        if (j > 0) {
+            SelectArgs sa{cb, B, rsel, h->slack, h->thr, h->ovf};
+            int e0 = pr.ev();
+            HIP_OR(launch_select_threshold(sa, st), false);
+            int e1 = pr.ev();
+            sel_ev.push_back({e0, e1});
+        }
+        ScanArgs a{};
+        a.kind = h->kind;
+        a.X = h->Xs;
+        a.xstride = h->xs_stride;
+        a.xn = h->xn;
+        a.row_begin = bounds[j];
+        a.row_end = bounds[j + 1];
+        a.d = h->d;
+        a.kdim = h->kdim;
+        a.Q = h->dQs;
+        a.qn = h->qn;
+        a.qscale = exact ? nullptr : h->qscale;
+        a.thr = h->thr;
+        a.B = B;
+        a.cand = cb;
+        int e0 = pr.ev();
+        const char* vname = nullptr;
+        HIP_OR(launch_scan(a, st, &vname), false);
+        int e1 = pr.ev();
+        scan_ev.push_back({e0, e1});
+        scan_rows.push_back((double)(a.row_end - a.row_begin));
+        scan_names.push_back(vname ? vname : "");
+    }
+    return finish_step(h, B, k, kk, rsel, ids_out, d_out, cb, exact, pr);
+}
+
+bool ivf_build(Handle* h);
+
 bool setup_mode(Handle* h) {
     hipStream_t st = h->st;
     const int64_t n = h->n;
     const int d = h->d;
+    if (h->ivf && !ivf_build(h)) return false;
     if (h->kind == SCAN_F32) {
         h->kdim = h->kdim32;
         h->Xs = h->X32;
@@ -437,13 +504,22 @@ bool setup_mode(Handle* h) {
         }
     }
     HIP_OR(launch_encode(h->kind, h->X32, h->kdim32, n, d, h->kdim, h->Xs, h->xn, nullptr, h->alpha_x, sc, of,
-                         h->p.sq_variant, st),
+                         h->p.sq_variant, st, h->ivf ? h->perm : nullptr),
            false);
     HIP_OR(hipStreamSynchronize(st), false);
+    if (h->ivf) {
+        // exact index over the centroids: the probe step (lsh_index_optimized.cpp:325-333)
+        annmi_params cp;
+        annmi_default_params(&cp, ANNMI_EXACT_F32);
+        cp.base_on_device = 1;
+        cp.max_batch = h->p.max_batch;
+        h->cent = create_impl(h->centroids, h->nlist, d, &cp, h->kdim32);
+        if (!h->cent) return false;
+    }
     return true;
 }
 
-Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params* params) {
+Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params* params, int64_t src_stride) {
     g_err[0] = 0;
     if (!base || n <= 0 || d <= 0) {
         set_err("empty index: n=%lld d=%d", (long long)n, d);
@@ -487,6 +563,7 @@ Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params*
     h->n = n;
     h->d = d;
     h->kind = kind;
+    h->ivf = p.mode == ANNMI_IVF_SQ8;
     h->kdim32 = round_up(d, 16);
     if (p.max_batch <= 0) h->p.max_batch = 4096;
     if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
@@ -501,7 +578,8 @@ Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params*
         return nullptr;
     };
     // fp32 rows (re-score base; the exact scan base)
-    if (p.base_on_device && h->kdim32 == d && (reinterpret_cast<uintptr_t>(base) & 15) == 0) {
+    if (p.base_on_device && h->kdim32 == d && (src_stride == 0 || src_stride == d) &&
+        (reinterpret_cast<uintptr_t>(base) & 15) == 0) {
         h->X32 = const_cast<float*>(base);
         h->own_x32 = false;
     } else {
@@ -512,7 +590,8 @@ Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params*
         h->own_x32 = true;
         h->bytes += n * (int64_t)h->kdim32 * 4;
         hipMemcpyKind kindc = p.base_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-        if (hipMemcpy2DAsync(h->X32, (size_t)h->kdim32 * 4, base, (size_t)d * 4, (size_t)d * 4, (size_t)n, kindc,
+        const size_t spitch = (size_t)(src_stride > 0 ? src_stride : d) * 4;
+        if (hipMemcpy2DAsync(h->X32, (size_t)h->kdim32 * 4, base, spitch, (size_t)d * 4, (size_t)n, kindc,
                              h->st) != hipSuccess) {
             set_err("copy of the base rows failed");
             return fail();
@@ -555,7 +634,7 @@ Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params*
 }
 
 int32_t batch_impl(Handle* h, const float* Q, int32_t B, int32_t k, int32_t R, int64_t* ids, float* dists,
-                   bool device) {
+                   bool device, int32_t nprobe) {
     g_err[0] = 0;
     if (!h) {
         set_err("null handle");
@@ -582,7 +661,7 @@ int32_t batch_impl(Handle* h, const float* Q, int32_t B, int32_t k, int32_t R, i
         HIP_OR(hipMemcpyAsync(h->dQ32, Q + (int64_t)s * h->d, (size_t)b * h->d * 4, kin, h->st), -1);
         int64_t* oid = device ? ids + (int64_t)s * k : h->d_ids;
         float* od = device ? dists + (int64_t)s * k : h->d_dist;
-        if (!run_step(h, b, k, R, oid, od)) return -1;
+        if (!run_step(h, b, k, R, nprobe, oid, od)) return -1;
         if (!device) {
             HIP_OR(hipMemcpyAsync(ids + (int64_t)s * k, h->d_ids, (size_t)b * k * 8, hipMemcpyDeviceToHost, h->st), -1);
             HIP_OR(hipMemcpyAsync(dists + (int64_t)s * k, h->d_dist, (size_t)b * k * 4, hipMemcpyDeviceToHost, h->st),
@@ -591,6 +670,267 @@ int32_t batch_impl(Handle* h, const float* Q, int32_t B, int32_t k, int32_t R, i
         }
     }
     return kk;
+}
+
+// ------------------------------------------------------------------ IVF --
+// k-means (lsh_index_optimized.cpp:551-616) on the GPU; see ivf.hip.
+bool ivf_build(Handle* h) {
+    hipStream_t st = h->st;
+    const int64_t n = h->n;
+    const int d = h->d, kd = h->kdim32;
+    int nlist = h->p.nlist > 0 ? h->p.nlist : 256;
+    if (nlist > n) nlist = (int)n;
+    const int iters = h->p.kmeans_iters > 0 ? h->p.kmeans_iters : 15;
+    const uint32_t seed = h->p.kmeans_seed > 0 ? (uint32_t)h->p.kmeans_seed : 1337u;
+    h->nlist = nlist;
+    // initial centroids: rows rand_indices[c % n] of shuffle(iota(n), mt19937(seed)) (:553-561)
+    std::vector<uint32_t> ri((size_t)n);
+    std::iota(ri.begin(), ri.end(), 0u);
+    {
+        std::vector<int> tmp((size_t)n);
+        std::iota(tmp.begin(), tmp.end(), 0);
+        std::shuffle(tmp.begin(), tmp.end(), std::mt19937(seed));
+        for (int64_t i = 0; i < n; ++i) ri[(size_t)i] = (uint32_t)tmp[(size_t)i];
+    }
+    std::vector<uint32_t> init((size_t)nlist);
+    for (int c = 0; c < nlist; ++c) init[(size_t)c] = ri[(size_t)(c % n)];
+    uint32_t *d_rand = nullptr, *d_init = nullptr, *assign = nullptr, *counts = nullptr, *offs = nullptr,
+             *keys_sorted = nullptr, *perm = nullptr, *iota = nullptr;
+    unsigned long long* am = nullptr;
+    float *cn = nullptr, *thr_inf = nullptr;
+    void *sort_tmp = nullptr, *scan_tmp = nullptr;
+    KmSeg* d_segs = nullptr;
+    int* d_segfirst = nullptr;
+    double* partial = nullptr;
+    struct Guard {
+        std::vector<void*> ptrs;
+        ~Guard() {
+            for (void* p : ptrs) dfree(p);
+        }
+    } guard;
+    auto alloc = [&](void** p, size_t bytes) -> bool {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            set_err("hipMalloc of %zu bytes failed in the IVF build", bytes);
+            return false;
+        }
+        guard.ptrs.push_back(*p);
+        return true;
+    };
+    if (!alloc((void**)&d_rand, n * 4) || !alloc((void**)&d_init, (size_t)nlist * 4) || !alloc((void**)&assign, n * 4) ||
+        !alloc((void**)&counts, (size_t)nlist * 4) || !alloc((void**)&offs, (size_t)nlist * 4) ||
+        !alloc((void**)&keys_sorted, n * 4) || !alloc((void**)&iota, n * 4) || !alloc((void**)&am, n * 8) ||
+        !alloc((void**)&cn, (size_t)nlist * 4) || !alloc((void**)&thr_inf, 4096 * 4))
+        return false;
+    HIP_OR(hipMalloc(&h->perm, n * 4), false);
+    perm = h->perm;
+    HIP_OR(hipMalloc(&h->centroids, (size_t)nlist * kd * 4), false);
+    h->bytes += n * 4 + (int64_t)nlist * kd * 4;
+    HIP_OR(hipMemcpyAsync(d_rand, ri.data(), n * 4, hipMemcpyHostToDevice, st), false);
+    HIP_OR(hipMemcpyAsync(d_init, init.data(), (size_t)nlist * 4, hipMemcpyHostToDevice, st), false);
+    HIP_OR(launch_gather_rows_f32(h->X32, kd, d_init, nlist, kd, h->centroids, st), false);
+    HIP_OR(launch_iota_u32(iota, n, st), false);
+    HIP_OR(launch_fill_f32(thr_inf, __builtin_inff(), 4096, st), false);
+    int key_bits = 1;
+    while ((1 << key_bits) < nlist) ++key_bits;
+    size_t sort_bytes = 0, scan_bytes = 0;
+    HIP_OR(sort_pairs_u32(nullptr, sort_bytes, assign, keys_sorted, iota, perm, n, key_bits, st), false);
+    HIP_OR(exclusive_sum_u32(nullptr, scan_bytes, counts, offs, nlist, st), false);
+    if (!alloc(&sort_tmp, sort_bytes + 16) || !alloc(&scan_tmp, scan_bytes + 16)) return false;
+    const int64_t kSeg = 8192;
+    const int max_segs = nlist + (int)(n / kSeg) + 1;
+    if (!alloc((void**)&d_segs, (size_t)max_segs * sizeof(KmSeg)) || !alloc((void**)&d_segfirst, (size_t)(nlist + 1) * 4) ||
+        !alloc((void**)&partial, (size_t)max_segs * d * 8))
+        return false;
+    std::vector<uint32_t> hoff((size_t)nlist), hcnt((size_t)nlist);
+    for (int it = 0; it < iters; ++it) {
+        // assign: argmin over centroids (scan.hip argmin mode), 4096 points per launch
+        HIP_OR(launch_encode(SCAN_F32, h->centroids, kd, nlist, d, kd, h->centroids, cn, nullptr, 1.f, nullptr, nullptr,
+                             0, st),
+               false);
+        HIP_OR(launch_fill_u64(am, ~0ull, n, st), false);
+        for (int64_t s0 = 0; s0 < n; s0 += 4096) {
+            const int b = (int)std::min<int64_t>(4096, n - s0);
+            ScanArgs a{};
+            a.kind = SCAN_F32;
+            a.X = h->centroids;
+            a.xstride = (int64_t)kd * 4;
+            a.xn = cn;
+            a.row_begin = 0;
+            a.row_end = nlist;
+            a.d = d;
+            a.kdim = kd;
+            a.Q = h->X32 + s0 * kd;
+            a.qn = h->xn32 + s0;
+            a.qscale = nullptr;
+            a.thr = thr_inf;
+            a.B = b;
+            a.cand = CandBuf{nullptr, nullptr, nullptr, 0};
+            a.argmin = am + s0;
+            HIP_OR(launch_scan(a, st), false);
+        }
+        HIP_OR(launch_fill_u32(counts, 0u, nlist, st), false);
+        HIP_OR(launch_argmin_to_assign(am, n, nlist, assign, counts, st), false);
+        // stable sort by cluster: the list layout, ascending row id inside a list
+        HIP_OR(sort_pairs_u32(sort_tmp, sort_bytes, assign, keys_sorted, iota, perm, n, key_bits, st), false);
+        HIP_OR(exclusive_sum_u32(scan_tmp, scan_bytes, counts, offs, nlist, st), false);
+        HIP_OR(hipMemcpyAsync(hoff.data(), offs, (size_t)nlist * 4, hipMemcpyDeviceToHost, st), false);
+        HIP_OR(hipMemcpyAsync(hcnt.data(), counts, (size_t)nlist * 4, hipMemcpyDeviceToHost, st), false);
+        HIP_OR(hipStreamSynchronize(st), false);
+        // update: fixed segments summed in order, then per-cluster means
+        std::vector<KmSeg> segs;
+        std::vector<int> first((size_t)nlist + 1);
+        for (int c = 0; c < nlist; ++c) {
+            first[(size_t)c] = (int)segs.size();
+            const int64_t b0 = hoff[(size_t)c], b1 = b0 + hcnt[(size_t)c];
+            for (int64_t p = b0; p < b1; p += kSeg) segs.push_back(KmSeg{p, std::min(b1, p + kSeg)});
+        }
+        first[(size_t)nlist] = (int)segs.size();
+        HIP_OR(hipMemcpyAsync(d_segs, segs.data(), segs.size() * sizeof(KmSeg), hipMemcpyHostToDevice, st), false);
+        HIP_OR(hipMemcpyAsync(d_segfirst, first.data(), first.size() * 4, hipMemcpyHostToDevice, st), false);
+        HIP_OR(launch_segment_sums(h->X32, kd, perm, d_segs, (int)segs.size(), d, partial, st), false);
+        HIP_OR(launch_centroid_update(partial, d_segfirst, counts, nlist, d, kd, h->X32, kd, d_rand, n, h->centroids, st),
+               false);
+        HIP_OR(hipStreamSynchronize(st), false);
+    }
+    h->list_off.assign((size_t)nlist + 1, 0);
+    for (int c = 0; c < nlist; ++c) h->list_off[(size_t)c] = hoff[(size_t)c];
+    h->list_off[(size_t)nlist] = n;
+    return true;
+}
+
+// IVF step: probe (exact search over the centroids), list scan of the
+// probed lists in two passes (a sample of every probed list with an open
+// filter, then the rest with the rsel-th smallest key as filter), finalize
+// (fp32 re-score of rsel candidates from the original rows, original ids).
+bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, float* d_out) {
+    hipStream_t st = h->st;
+    const int kk = (int)std::min<int64_t>(k, h->n);
+    int rsel = std::max(kk, R);  // refine_r, at least k (:472-474)
+    rsel = (int)std::min<int64_t>(rsel, h->n);
+    if (rsel > 2048) {
+        set_err("rerank R=%d exceeds the 2048-row re-score limit", rsel);
+        return false;
+    }
+    if (nprobe <= 0) nprobe = h->p.nprobe > 0 ? h->p.nprobe : 8;
+    nprobe = std::min(nprobe, h->nlist);
+    std::vector<std::pair<int, int>> scan_ev, sel_ev;
+    std::vector<double> scan_rows;
+    std::vector<std::string> scan_names;
+    int ev_used = 0;
+    Profiler pr{h, ev_used, &scan_ev, &sel_ev, &scan_rows, &scan_names, -1, -1};
+    pr.ev_prep0 = pr.ev();
+    HIP_OR(launch_prep_queries(SCAN_SQ8, h->dQ32, B, h->d, h->kdim, h->dQs, h->qn, h->qscale, h->slack, h->maxnorm,
+                               1.f, h->sq_scale, h->sq_offset, st),
+           false);
+    HIP_OR(hipStreamSynchronize(st), false);
+    // probe lists: nprobe nearest centroids, exact order, lower id on ties
+    if (h->probe_cap < h->ws_B * nprobe) {
+        dfree(h->d_probe);
+        dfree(h->d_probe_d);
+        h->d_probe = nullptr;
+        h->d_probe_d = nullptr;
+        h->probe_cap = h->ws_B * nprobe;
+        HIP_OR(hipMalloc(&h->d_probe, (size_t)h->probe_cap * 8), false);
+        HIP_OR(hipMalloc(&h->d_probe_d, (size_t)h->probe_cap * 4), false);
+    }
+    if (batch_impl(h->cent, h->dQ32, B, nprobe, 0, h->d_probe, h->d_probe_d, true, 0) < 0) return false;
+    std::vector<int64_t> probe((size_t)B * nprobe);
+    HIP_OR(hipMemcpy(probe.data(), h->d_probe, probe.size() * 8, hipMemcpyDeviceToHost), false);
+    // work items: (list chunk x group of <= 16 queries probing that list)
+    std::vector<std::vector<uint32_t>> qlist((size_t)h->nlist);
+    std::vector<int64_t> scanned((size_t)B, 0);
+    for (int q = 0; q < B; ++q)
+        for (int j = 0; j < nprobe; ++j) {
+            int64_t l = probe[(size_t)q * nprobe + j];
+            if (l < 0) continue;
+            qlist[(size_t)l].push_back((uint32_t)q);
+            scanned[(size_t)q] += h->list_off[(size_t)l + 1] - h->list_off[(size_t)l];
+        }
+    const int64_t S = std::max<int64_t>(256, (4 * (int64_t)rsel + nprobe - 1) / nprobe);
+    const uint32_t CH = 256;
+    std::vector<uint32_t> gq;
+    std::vector<IvfItem> itA, itB;
+    double bytesA = 0, bytesB = 0, flopsA = 0, flopsB = 0;
+    for (int l = 0; l < h->nlist; ++l) {
+        const auto& ql = qlist[(size_t)l];
+        if (ql.empty()) continue;
+        const uint32_t b0 = (uint32_t)h->list_off[(size_t)l], b1 = (uint32_t)h->list_off[(size_t)l + 1];
+        const uint32_t sA = (uint32_t)std::min<int64_t>(b1, b0 + S);
+        for (size_t g0 = 0; g0 < ql.size(); g0 += 16) {
+            const uint32_t nq = (uint32_t)std::min<size_t>(16, ql.size() - g0);
+            const uint32_t qoff = (uint32_t)gq.size();
+            for (uint32_t t = 0; t < nq; ++t) gq.push_back(ql[g0 + t]);
+            for (uint32_t r = b0; r < sA; r += CH) {
+                itA.push_back(IvfItem{r, std::min(sA, r + CH), qoff, nq});
+                bytesA += (double)(std::min(sA, r + CH) - r) * h->xs_stride;
+                flopsA += 2.0 * (std::min(sA, r + CH) - r) * nq * h->d;
+            }
+            for (uint32_t r = sA; r < b1; r += CH) {
+                itB.push_back(IvfItem{r, std::min(b1, r + CH), qoff, nq});
+                bytesB += (double)(std::min(b1, r + CH) - r) * h->xs_stride;
+                flopsB += 2.0 * (std::min(b1, r + CH) - r) * nq * h->d;
+            }
+        }
+    }
+    const int nA = (int)itA.size(), nB = (int)itB.size();
+    if (h->items_cap < nA + nB) {
+        dfree(h->d_items);
+        h->d_items = nullptr;
+        h->items_cap = std::max(nA + nB, 1024);
+        HIP_OR(hipMalloc(&h->d_items, (size_t)h->items_cap * sizeof(IvfItem)), false);
+    }
+    if (h->gq_cap < (int64_t)gq.size() || !h->d_gq) {
+        dfree(h->d_gq);
+        h->d_gq = nullptr;
+        h->gq_cap = std::max<int64_t>((int64_t)gq.size(), 4096);
+        HIP_OR(hipMalloc(&h->d_gq, (size_t)h->gq_cap * 4), false);
+    }
+    if (!h->d_next) HIP_OR(hipMalloc(&h->d_next, 2 * sizeof(uint32_t)), false);
+    itA.insert(itA.end(), itB.begin(), itB.end());
+    if (!itA.empty())
+        HIP_OR(hipMemcpyAsync(h->d_items, itA.data(), itA.size() * sizeof(IvfItem), hipMemcpyHostToDevice, st), false);
+    if (!gq.empty()) HIP_OR(hipMemcpyAsync(h->d_gq, gq.data(), gq.size() * 4, hipMemcpyHostToDevice, st), false);
+    HIP_OR(launch_fill_u32(h->cnt, 0u, B, st), false);
+    HIP_OR(launch_fill_u32(h->ovf, 0u, B, st), false);
+    HIP_OR(launch_fill_f32(h->thr, __builtin_inff(), B, st), false);
+    HIP_OR(launch_fill_u32(h->d_next, 0u, 2, st), false);
+    CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
+    pr.ev_prep1 = pr.ev();
+    IvfScanArgs ia{};
+    ia.codes = h->Xs;
+    ia.xstride = h->xs_stride;
+    ia.xn = h->xn;
+    ia.perm = h->perm;
+    ia.kdim = h->kdim;
+    ia.Q = h->dQs;
+    ia.qn = h->qn;
+    ia.qscale = h->qscale;
+    ia.thr = h->thr;
+    ia.gq = h->d_gq;
+    ia.cand = cb;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            SelectArgs sa{cb, B, rsel, h->slack, h->thr, h->ovf};
+            int e0 = pr.ev();
+            HIP_OR(launch_select_threshold(sa, st), false);
+            int e1 = pr.ev();
+            sel_ev.push_back({e0, e1});
+        }
+        ia.items = h->d_items + (pass == 0 ? 0 : nA);
+        ia.nitems = pass == 0 ? nA : nB;
+        ia.next = h->d_next + pass;
+        int e0 = pr.ev();
+        HIP_OR(launch_ivf_scan(ia, st), false);
+        int e1 = pr.ev();
+        scan_ev.push_back({e0, e1});
+        scan_rows.push_back(0.0);
+        scan_names.push_back(pass == 0 ? "ivf_scan_kernel<sample>" : "ivf_scan_kernel<lists>");
+        pr.bytes_override.push_back(pass == 0 ? bytesA : bytesB);
+        pr.flops_override.push_back(pass == 0 ? flopsA : flopsB);
+    }
+    if (!finish_step(h, B, k, kk, rsel, ids_out, d_out, cb, false, pr)) return false;
+    h->last_scanned.insert(h->last_scanned.end(), scanned.begin(), scanned.end());
+    return true;
 }
 
 }  // namespace
@@ -605,24 +945,33 @@ void annmi_default_params(annmi_params* p, int32_t mode) {
     p->sq_variant = 0;
     p->base_on_device = 0;
     p->max_batch = 4096;
+    p->nlist = 256;
+    p->nprobe = 8;
+    p->kmeans_iters = 15;
+    p->kmeans_seed = 1337;
 }
 
 void* annmi_create(const float* base, int64_t n, int32_t d, const annmi_params* params) {
-    return create_impl(base, n, d, params);
+    return create_impl(base, n, d, params, 0);
 }
 
 int32_t annmi_query(void* h, const float* q, int32_t k, int32_t rerank, int64_t* out_ids, float* out_dists) {
-    return batch_impl(static_cast<Handle*>(h), q, 1, k, rerank, out_ids, out_dists, false);
+    return batch_impl(static_cast<Handle*>(h), q, 1, k, rerank, out_ids, out_dists, false, 0);
 }
 
 int32_t annmi_batch_query(void* h, const float* Q, int32_t B, int32_t k, int32_t rerank, int64_t* out_ids,
                           float* out_dists) {
-    return batch_impl(static_cast<Handle*>(h), Q, B, k, rerank, out_ids, out_dists, false);
+    return batch_impl(static_cast<Handle*>(h), Q, B, k, rerank, out_ids, out_dists, false, 0);
+}
+
+int32_t annmi_batch_query_ex(void* h, const float* Q, int32_t B, int32_t k, int32_t rerank, int32_t nprobe,
+                             int64_t* out_ids, float* out_dists, int32_t on_device) {
+    return batch_impl(static_cast<Handle*>(h), Q, B, k, rerank, out_ids, out_dists, on_device != 0, nprobe);
 }
 
 int32_t annmi_batch_query_device(void* h, const float* dQ, int32_t B, int32_t k, int32_t rerank, int64_t* d_ids,
                                  float* d_dists) {
-    return batch_impl(static_cast<Handle*>(h), dQ, B, k, rerank, d_ids, d_dists, true);
+    return batch_impl(static_cast<Handle*>(h), dQ, B, k, rerank, d_ids, d_dists, true, 0);
 }
 
 int64_t annmi_ndist(void* h) { return h ? static_cast<Handle*>(h)->ndist.load() : 0; }
@@ -662,6 +1011,12 @@ int64_t annmi_export(void* hv, int32_t what, void* out, int64_t bytes) {
         case 1: src = h->sq_scale; need = h->sq_scale ? (int64_t)h->kdim * 4 : 0; break;
         case 2: src = h->sq_offset; need = h->sq_offset ? (int64_t)h->kdim * 4 : 0; break;
         case 3: src = h->xn; need = h->n * 4; break;
+        case 5: src = h->perm; need = h->perm ? h->n * 4 : 0; break;
+        case 6:
+            need = (int64_t)h->list_off.size() * 8;
+            if (out && bytes >= need && need) std::memcpy(out, h->list_off.data(), (size_t)need);
+            return need;
+        case 7: src = h->centroids; need = h->centroids ? (int64_t)h->nlist * h->kdim32 * 4 : 0; break;
         case 4: {
             // scalar parameters: kdim, row stride (bytes), fp8 alpha, max row norm
             if (out && bytes >= 32) {
@@ -725,7 +1080,7 @@ void* gf_create(const float* train, long n, long dim) {
         set_err("dim %ld > %d unsupported", dim, kMaxDim);
         return nullptr;
     }
-    return create_impl(train, (int64_t)n, (int32_t)dim, &p);
+    return create_impl(train, (int64_t)n, (int32_t)dim, &p, 0);
 }
 
 int gf_query(void* handle, const float* q, int k, int r, long long* out_ids) {
@@ -739,7 +1094,7 @@ int gf_query(void* handle, const float* q, int k, int r, long long* out_ids) {
     if (r > h->n) r = (int)h->n;
     std::vector<float> dd(k);
     static_assert(sizeof(long long) == sizeof(int64_t), "");
-    return batch_impl(h, q, 1, k, r, reinterpret_cast<int64_t*>(out_ids), dd.data(), false);
+    return batch_impl(h, q, 1, k, r, reinterpret_cast<int64_t*>(out_ids), dd.data(), false, 0);
 }
 
 long long gf_ndist(void* handle) { return handle ? (long long)static_cast<Handle*>(handle)->ndist.load() : 0; }

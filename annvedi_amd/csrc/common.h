@@ -72,6 +72,12 @@ struct ScanArgs {
     const float* thr;         // [B]
     int B;
     CandBuf cand;
+    // argmin mode (k-means assignment): when set, no candidates are appended;
+    // argmin[q] receives min over rows of (f2key(dist) << 32 | row) via a
+    // 64-bit atomicMin (smallest distance, then lowest row).
+    unsigned long long* argmin = nullptr;
+    // candidate ids are rid_map[row] when set (list-ordered rows -> original ids)
+    const uint32_t* rid_map = nullptr;
 };
 // Launches the scan; *variant (if not null) receives a static name of the
 // kernel instantiation launched (for per-kernel timing).

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void enc_f32(const float* src, int64_t sstride
     s = warp16_sum(s);
     if (l == 0) {
         xn[row] = s;
-        atomicMax(maxkey, f2key(sqrtf(s)));
+        if (maxkey) atomicMax(maxkey, f2key(sqrtf(s)));
     }
 }
 
@@ -95,11 +95,11 @@ __device__ inline int sq_round(float t, int variant) {
 
 __global__ __launch_bounds__(256) void enc_sq8(const float* src, int64_t sstride, int64_t n, int d, int kdim,
                                                const float* scale, const float* offset, int variant, uint8_t* dst,
-                                               float* xn) {
+                                               float* xn, const uint32_t* gather) {
     const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
     const int l = threadIdx.x & 15;
     if (row >= n) return;
-    const float* p = src + row * sstride;
+    const float* p = src + (gather ? (int64_t)gather[row] : row) * sstride;
     float s = 0.f;
     for (int j = l; j < kdim; j += 16) {
         uint8_t c = 0;
@@ -315,7 +315,8 @@ __global__ __launch_bounds__(256) void prep_queries(QPrepArgs a) {
 
 hipError_t launch_encode(int kind, const float* src, int64_t sstride, int64_t n, int d, int kdim, void* dst, float* xn,
                          uint32_t* maxkey, float alpha, const float* scale, const float* offset, int variant,
-                         hipStream_t st) {
+                         hipStream_t st, const uint32_t* gather) {
+    if (gather && kind != SCAN_SQ8) return hipErrorInvalidValue;
     if (n <= 0) return hipSuccess;
     dim3 grid((unsigned)((n + 15) / 16));
     switch (kind) {
@@ -333,7 +334,7 @@ hipError_t launch_encode(int kind, const float* src, int64_t sstride, int64_t n,
             break;
         case SCAN_SQ8:
             hipLaunchKernelGGL(enc_sq8, grid, dim3(256), 0, st, src, sstride, n, d, kdim, scale, offset, variant,
-                               static_cast<uint8_t*>(dst), xn);
+                               static_cast<uint8_t*>(dst), xn, gather);
             break;
         case SCAN_SQ4:
             hipLaunchKernelGGL(enc_sq4, grid, dim3(256), 0, st, src, sstride, n, d, kdim, scale, offset,

@@ -1,0 +1,230 @@
+// IVF index: k-means build kernels and the list-scan kernel.
+//
+// Build (competitors/ivf_lsh/src/lsh_index_optimized.cpp:551-616, reorder
+// :155-173): centroids start at rows of a std::shuffle(iota, mt19937(1337))
+// permutation (done on the host), 15 rounds of assign + mean update, an
+// empty cluster restarts at row rand_indices[(c*37) % n].  Assignment is the
+// argmin scan (scan.hip, argmin mode: smallest fp32 distance, then lowest
+// centroid id).  The update is deterministic: points are sorted by (cluster,
+// id) with a stable radix sort, each cluster's rows are summed in that order
+// in fp64 over fixed segments, segments are summed in order.  The list layout
+// is the sorted order (cluster-contiguous, ascending original id inside a
+// cluster, as the reference's cluster_to_points push_back order).
+//
+// Query: the list-scan kernel walks work items (row chunk of one list x
+// group of up to 16 queries that probe that list), pulled from an atomic
+// queue by every wave, so Zipf-skewed list lengths only change the number of
+// items.  Each item is one 16-query x 16-row int8 MFMA tile stream over the
+// list's SQ8 codes, with the flat scan's threshold filter and append.
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+#include "ivf.h"
+
+namespace {
+
+__global__ void gather_rows_f32(const float* src, int64_t stride, const uint32_t* idx, int64_t n, int kdim,
+                                float* dst) {
+    const int64_t i = blockIdx.x;
+    if (i >= n) return;
+    const float* s = src + (int64_t)idx[i] * stride;
+    for (int j = threadIdx.x; j < kdim; j += blockDim.x) dst[i * kdim + j] = s[j];
+}
+
+__global__ void fill_u64(unsigned long long* p, unsigned long long v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void iota_u32(uint32_t* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+
+__global__ void argmin_to_assign(const unsigned long long* am, int64_t n, int nlist, uint32_t* assign,
+                                 uint32_t* counts) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c = (uint32_t)(am[i] & 0xFFFFFFFFull);
+    if (c >= (uint32_t)nlist) c = 0;  // a row whose distances were all NaN
+    assign[i] = c;
+    atomicAdd(&counts[c], 1u);
+}
+
+// One workgroup per segment [pos_begin, pos_end) of one cluster's sorted
+// points: fp64 column sums in sorted order.
+__global__ __launch_bounds__(256) void segment_sums(const float* X, int64_t stride, const uint32_t* perm,
+                                                   const KmSeg* segs, int d, double* partial) {
+    const KmSeg s = segs[blockIdx.x];
+    double* out = partial + (int64_t)blockIdx.x * d;
+    for (int j0 = 0; j0 < d; j0 += 256) {
+        const int j = j0 + threadIdx.x;
+        double acc = 0.0;
+        if (j < d)
+            for (int64_t p = s.pos_begin; p < s.pos_end; ++p) acc += (double)X[(int64_t)perm[p] * stride + j];
+        if (j < d) out[j] = acc;
+    }
+}
+
+// centroid[c] = (sum of its segments, in order) / count; empty cluster ->
+// row rand_idx[(c * 37) % n] (lsh_index_optimized.cpp:609-613).
+__global__ __launch_bounds__(256) void centroid_update(const double* partial, const int* seg_first,
+                                                       const uint32_t* counts, int d, int kdim, const float* X,
+                                                       int64_t stride, const uint32_t* rand_idx, int64_t n,
+                                                       float* cent) {
+    const int c = blockIdx.x;
+    const uint32_t cnt = counts[c];
+    float* o = cent + (int64_t)c * kdim;
+    if (cnt == 0) {
+        const float* src = X + (int64_t)rand_idx[((int64_t)c * 37) % n] * stride;
+        for (int j = threadIdx.x; j < kdim; j += 256) o[j] = j < d ? src[j] : 0.f;
+        return;
+    }
+    const int s0 = seg_first[c], s1 = seg_first[c + 1];
+    for (int j = threadIdx.x; j < kdim; j += 256) {
+        if (j >= d) {
+            o[j] = 0.f;
+            continue;
+        }
+        double acc = 0.0;
+        for (int s = s0; s < s1; ++s) acc += partial[(int64_t)s * d + j];
+        o[j] = (float)(acc / (double)cnt);
+    }
+}
+
+// ------------------------------------------------------------ list scan --
+constexpr int kIvfThreads = 256;
+
+__global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(IvfScanArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4;
+    const int n = lane & 15;
+    const uint8_t* X = static_cast<const uint8_t*>(a.codes);
+    const int nchunk = a.kdim / 64;
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(a.next, 1u);
+        it = __shfl(it, 0);
+        if (it >= (uint32_t)a.nitems) break;
+        const IvfItem item = a.items[it];
+        // A operand: query n of the group (16 queries per tile)
+        const int qa = n < (int)item.nq ? (int)a.gq[item.qoff + n] : -1;
+        // epilogue slots: queries 4g + i of the group
+        int qe[4];
+        float thr[4], qn[4], qs[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = 4 * g + i;
+            qe[i] = m < (int)item.nq ? (int)a.gq[item.qoff + m] : -1;
+            thr[i] = qe[i] >= 0 ? a.thr[qe[i]] : -__builtin_inff();
+            qn[i] = qe[i] >= 0 ? a.qn[qe[i]] : 0.f;
+            qs[i] = qe[i] >= 0 ? a.qscale[qe[i]] : 0.f;
+        }
+        const uint8_t* qrow = qa >= 0 ? static_cast<const uint8_t*>(a.Q) + (int64_t)qa * a.kdim : nullptr;
+        for (uint32_t r0 = item.row_begin; r0 < item.row_end; r0 += 16) {
+            const uint32_t row = r0 + n;
+            const bool rok = row < item.row_end;
+            const int64_t lrow = rok ? row : item.row_end - 1;
+            i32x4 acc = (i32x4){0, 0, 0, 0};
+            for (int c = 0; c < nchunk; ++c) {
+                i32x4 b = *reinterpret_cast<const i32x4*>(X + lrow * a.xstride + c * 64 + 16 * g);
+                b ^= (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
+                i32x4 q = qrow ? *reinterpret_cast<const i32x4*>(qrow + c * 64 + 16 * g) : (i32x4){0, 0, 0, 0};
+                acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(q, b, acc, 0, 0, 0);
+            }
+            const float xn = a.xn[lrow];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float dist = xn + qn[i] - 2.0f * qs[i] * (float)acc[i];
+                const bool pass = rok && dist <= thr[i];
+                uint64_t mb = __ballot(pass);
+                if (mb == 0) continue;
+                uint32_t gm = (uint32_t)(mb >> (16 * g)) & 0xFFFFu;
+                uint32_t base = 0;
+                if (n == 0 && gm) base = atomicAdd(&a.cand.cnt[qe[i]], (uint32_t)__builtin_popcount(gm));
+                base = __shfl(base, lane & ~15);
+                if (pass) {
+                    uint32_t off = base + __builtin_popcount(gm & ((1u << n) - 1u));
+                    if (off < (uint32_t)a.cand.cap) {
+                        int64_t sl = (int64_t)qe[i] * a.cand.cap + off;
+                        a.cand.key[sl] = dist;
+                        a.cand.rid[sl] = a.perm[row];
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_gather_rows_f32(const float* src, int64_t stride, const uint32_t* idx, int64_t n, int kdim,
+                                  float* dst, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_f32, dim3((unsigned)n), dim3(256), 0, st, src, stride, idx, n, kdim, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota_u32(uint32_t* p, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(iota_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_argmin_to_assign(const unsigned long long* am, int64_t n, int nlist, uint32_t* assign,
+                                   uint32_t* counts, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(argmin_to_assign, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, am, n, nlist, assign,
+                       counts);
+    return hipGetLastError();
+}
+
+hipError_t sort_pairs_u32(void* tmp, size_t& tmp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                          const uint32_t* vals_in, uint32_t* vals_out, int64_t n, int key_bits, hipStream_t st) {
+    return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out, (int)n, 0,
+                                              key_bits, st);
+}
+
+hipError_t exclusive_sum_u32(void* tmp, size_t& tmp_bytes, const uint32_t* in, uint32_t* out, int64_t n,
+                             hipStream_t st) {
+    return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, (int)n, st);
+}
+
+hipError_t launch_segment_sums(const float* X, int64_t stride, const uint32_t* perm, const KmSeg* segs, int nseg,
+                               int d, double* partial, hipStream_t st) {
+    if (nseg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(segment_sums, dim3((unsigned)nseg), dim3(256), 0, st, X, stride, perm, segs, d, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_centroid_update(const double* partial, const int* seg_first, const uint32_t* counts, int nlist,
+                                  int d, int kdim, const float* X, int64_t stride, const uint32_t* rand_idx,
+                                  int64_t n, float* cent, hipStream_t st) {
+    if (nlist <= 0) return hipSuccess;
+    hipLaunchKernelGGL(centroid_update, dim3((unsigned)nlist), dim3(256), 0, st, partial, seg_first, counts, d, kdim,
+                       X, stride, rand_idx, n, cent);
+    return hipGetLastError();
+}
+
+hipError_t launch_ivf_scan(const IvfScanArgs& a, hipStream_t st) {
+    if (a.nitems <= 0) return hipSuccess;
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 256;
+    }
+    // persistent: a few workgroups per CU, every wave pulls items until the queue is empty
+    int64_t want = (int64_t)ncu * 8;
+    int64_t need = (a.nitems + 3) / 4;
+    unsigned grid = (unsigned)(want < need ? want : need);
+    hipLaunchKernelGGL(ivf_scan_kernel, dim3(grid), dim3(kIvfThreads), 0, st, a);
+    return hipGetLastError();
+}

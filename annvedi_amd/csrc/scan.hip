@@ -128,6 +128,25 @@ __host__ __device__ inline int qrow_bytes(int kdim) {
     return b + 16;
 }
 
+// Reduce the per-lane running (dist, row) minima over the 16 lanes that hold
+// the same queries (different rows) and publish them with a 64-bit atomicMin.
+template <int QS>
+__device__ inline void argmin_flush(unsigned long long (&best)[QS][4], const ScanArgs& a, int q0, int g, int n) {
+#pragma unroll
+    for (int w = 0; w < QS; ++w)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            unsigned long long v = best[w][i];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                unsigned long long u = __shfl_xor(v, o, 16);
+                v = u < v ? u : v;
+            }
+            const int qi = q0 + 16 * w + 4 * g + i;
+            if (n == 0 && qi < a.B && v != ~0ull) atomicMin(&a.argmin[qi], v);
+        }
+}
+
 template <class P, int QW>
 __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -170,6 +189,11 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
     const uint8_t* X = static_cast<const uint8_t*>(a.X);
     const int nchunk = a.kdim / P::kChunk;
     const int lane_dim = P::kDimsPerLoad * g;  // dims offset of this lane group inside a chunk
+    unsigned long long best[QW][4];
+#pragma unroll
+    for (int w = 0; w < QW; ++w)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) best[w][i] = ~0ull;
 
     for (int64_t t = t_begin + wave; t < t_end; t += 4) {
         const int64_t row = a.row_begin + t * 16 + n;
@@ -209,6 +233,17 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
 
         // epilogue: distance, filter, append
         const float xn = a.xn[lrow];
+        if (a.argmin) {
+#pragma unroll
+            for (int w = 0; w < QW; ++w)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float dist = xn + qn[w][i] - 2.0f * qs[w][i] * (float)acc[w][i];
+                    unsigned long long kv = ((unsigned long long)f2key(dist) << 32) | (uint32_t)row;
+                    if (row_ok && kv < best[w][i]) best[w][i] = kv;
+                }
+            continue;
+        }
 #pragma unroll
         for (int w = 0; w < QW; ++w) {
 #pragma unroll
@@ -229,12 +264,13 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                     if (off < (uint32_t)a.cand.cap) {
                         int64_t slot = (int64_t)qi * a.cand.cap + off;
                         a.cand.key[slot] = dist;
-                        a.cand.rid[slot] = (uint32_t)row;
+                        a.cand.rid[slot] = a.rid_map ? a.rid_map[row] : (uint32_t)row;
                     }
                 }
             }
         }
     }
+    if (a.argmin) argmin_flush<QW>(best, a, q0, g, n);
 }
 
 template <class P>
@@ -324,6 +360,11 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     const int lane_dim = P::kDimsPerLoad * g;
     constexpr int TR = 16 * RS;  // rows per wave step
     const int64_t ntiles = (r_end - r_begin + TR - 1) / TR;
+    unsigned long long best[QS][4];
+#pragma unroll
+    for (int q = 0; q < QS; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) best[q][i] = ~0ull;
 
     for (int64_t t = wave; t < ntiles; t += kWideWaves) {
         const int64_t r0 = r_begin + t * TR;
@@ -362,6 +403,22 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
         float xn[RS];
 #pragma unroll
         for (int r = 0; r < RS; ++r) xn[r] = a.xn[lrow[r]];
+        if (a.argmin) {
+#pragma unroll
+            for (int q = 0; q < QS; ++q)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int m = 16 * q + 4 * g + i;
+                    const float qv = qn_s[m], sc = 2.0f * qsc_s[m];
+#pragma unroll
+                    for (int r = 0; r < RS; ++r) {
+                        float dist = xn[r] + qv - sc * (float)acc[q][r][i];
+                        unsigned long long kv = ((unsigned long long)f2key(dist) << 32) | (uint32_t)(r0 + 16 * r + n);
+                        if (rok[r] && kv < best[q][i]) best[q][i] = kv;
+                    }
+                }
+            continue;
+        }
         bool any = false;
         v4f dist4[QS][RS];
 #pragma unroll
@@ -399,13 +456,15 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                         uint32_t off = base + __builtin_popcount(gm & ((1u << n) - 1u));
                         if (off < (uint32_t)a.cand.cap) {
                             int64_t sl = (int64_t)qi * a.cand.cap + off;
+                            const uint32_t row = (uint32_t)(r0 + 16 * r + n);
                             a.cand.key[sl] = dist;
-                            a.cand.rid[sl] = (uint32_t)(r0 + 16 * r + n);
+                            a.cand.rid[sl] = a.rid_map ? a.rid_map[row] : row;
                         }
                     }
                 }
             }
     }
+    if (a.argmin) argmin_flush<QS>(best, a, q0, g, n);
 }
 
 template <class P, int QS, int RS, int NW>

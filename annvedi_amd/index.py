@@ -15,7 +15,8 @@ from . import _lib
 
 
 class Index:
-    def __init__(self, base, mode: str = "exact", rerank: int = 0, sq_variant: int = 0, max_batch: int = 4096):
+    def __init__(self, base, mode: str = "exact", rerank: int = 0, sq_variant: int = 0, max_batch: int = 4096,
+                 nlist: int = 0, nprobe: int = 0, kmeans_iters: int = 0, kmeans_seed: int = 0):
         L = _lib.load()
         if mode not in _lib.MODES:
             raise ValueError(f"unknown mode {mode!r}; one of {sorted(_lib.MODES)}")
@@ -24,6 +25,14 @@ class Index:
         p.rerank = int(rerank)
         p.sq_variant = int(sq_variant)
         p.max_batch = int(max_batch)
+        if nlist:
+            p.nlist = int(nlist)
+        if nprobe:
+            p.nprobe = int(nprobe)
+        if kmeans_iters:
+            p.kmeans_iters = int(kmeans_iters)
+        if kmeans_seed:
+            p.kmeans_seed = int(kmeans_seed)
         self.mode = mode
         self._keep = None
         if _is_torch_cuda(base):
@@ -56,7 +65,7 @@ class Index:
         if Q.ndim != 2 or Q.shape[1] != self.d:
             raise RuntimeError(f"queries must have shape (B, {self.d}), got {tuple(Q.shape)}")
 
-    def search(self, Q, k: int, rerank: int = 0):
+    def search(self, Q, k: int, rerank: int = 0, nprobe: int = 0):
         """(ids int64 [B, k], dists float32 [B, k]) for a host batch."""
         Q = np.ascontiguousarray(np.asarray(Q, dtype=np.float32))
         if Q.ndim == 1:
@@ -65,14 +74,14 @@ class Index:
         B = Q.shape[0]
         ids = np.empty((B, k), dtype=np.int64)
         dd = np.empty((B, k), dtype=np.float32)
-        got = self._L.annmi_batch_query(self._h, Q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), B, int(k),
-                                        int(rerank), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-                                        dd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        got = self._L.annmi_batch_query_ex(self._h, Q.ctypes.data_as(ctypes.c_void_p), B, int(k), int(rerank),
+                                           int(nprobe), ids.ctypes.data_as(ctypes.c_void_p),
+                                           dd.ctypes.data_as(ctypes.c_void_p), 0)
         if got < 0:
             raise RuntimeError(_lib.last_error())
         return ids[:, :got], dd[:, :got]
 
-    def search_device(self, Q, k: int, rerank: int = 0):
+    def search_device(self, Q, k: int, rerank: int = 0, nprobe: int = 0):
         """Same for a torch device tensor; returns device tensors."""
         import torch
 
@@ -82,8 +91,9 @@ class Index:
         ids = torch.empty((B, k), dtype=torch.int64, device=Q.device)
         dd = torch.empty((B, k), dtype=torch.float32, device=Q.device)
         torch.cuda.current_stream().synchronize()
-        got = self._L.annmi_batch_query_device(self._h, ctypes.c_void_p(Q.data_ptr()), B, int(k), int(rerank),
-                                               ctypes.c_void_p(ids.data_ptr()), ctypes.c_void_p(dd.data_ptr()))
+        got = self._L.annmi_batch_query_ex(self._h, ctypes.c_void_p(Q.data_ptr()), B, int(k), int(rerank),
+                                           int(nprobe), ctypes.c_void_p(ids.data_ptr()),
+                                           ctypes.c_void_p(dd.data_ptr()), 1)
         if got < 0:
             raise RuntimeError(_lib.last_error())
         return ids[:, :got], dd[:, :got]
@@ -99,7 +109,8 @@ class Index:
         """Host copy of index state: 'rows' (scan rows as uint8 [n, stride]),
         'scale', 'offset' (SQ per-dim, float32), 'norms' (float32 [n]),
         'meta' (dict)."""
-        codes = {"rows": 0, "scale": 1, "offset": 2, "norms": 3, "meta": 4}
+        codes = {"rows": 0, "scale": 1, "offset": 2, "norms": 3, "meta": 4, "perm": 5, "list_offsets": 6,
+                 "centroids": 7}
         w = codes[what]
         need = self._L.annmi_export(self._h, w, None, 0)
         if need < 0:
@@ -114,6 +125,13 @@ class Index:
             raise RuntimeError(_lib.last_error())
         if w in (1, 2, 3):
             return buf.view(np.float32)
+        if w == 5:
+            return buf.view(np.uint32)
+        if w == 6:
+            return buf.view(np.int64)
+        if w == 7:
+            kd = (self.d + 15) // 16 * 16
+            return buf.view(np.float32).reshape(-1, kd)[:, :self.d]
         stride = self.export("meta")["row_stride"]
         return buf.reshape(self.n, stride)
 

@@ -23,7 +23,9 @@ enum {
     ANNMI_BF16 = 1,      /* bf16 MFMA candidates + fp32 re-score of R                       */
     ANNMI_FP8 = 2,       /* e4m3 MFMA candidates + fp32 re-score of R                       */
     ANNMI_SQ8 = 3,       /* per-dim SQ8 codes, int8 MFMA + fp32 re-score (gpu_filter.cu)    */
-    ANNMI_SQ4 = 4        /* per-dim SQ4 nibbles, int8 MFMA + fp32 re-score                  */
+    ANNMI_SQ4 = 4,       /* per-dim SQ4 nibbles, int8 MFMA + fp32 re-score                  */
+    ANNMI_IVF_SQ8 = 5    /* k-means IVF, per-dim SQ8 codes per list, list scan + fp32 re-score
+                            (competitors/ivf_lsh/src/lsh_index_optimized.cpp)               */
 };
 
 typedef struct annmi_params {
@@ -34,7 +36,11 @@ typedef struct annmi_params {
                                constant dim offset min)                                          */
     int32_t base_on_device; /* 1: `base` is a device pointer, borrowed for the handle's life    */
     int64_t max_batch;      /* queries per internal step (0 -> 4096)                           */
-    int64_t reserved[4];
+    int32_t nlist;          /* IVF lists (0 -> 256, lsh_index_optimized.cpp:128 default)       */
+    int32_t nprobe;         /* IVF default lists probed per query (0 -> 8, :304 default)       */
+    int32_t kmeans_iters;   /* IVF k-means rounds (0 -> 15, :564)                              */
+    int32_t kmeans_seed;    /* IVF initial-centroid shuffle seed (0 -> 1337, :555)             */
+    int64_t reserved[2];
 } annmi_params;
 
 /* Fills `p` with the defaults for `mode`. */
@@ -58,6 +64,13 @@ int32_t annmi_query(void* h, const float* q, int32_t k, int32_t rerank, int64_t*
  * orthogonal-competition/harness.py:159-174).  Returns min(k, n) or -1. */
 int32_t annmi_batch_query(void* h, const float* Q, int32_t B, int32_t k, int32_t rerank, int64_t* out_ids,
                           float* out_dists);
+
+/* B queries with an explicit IVF probe count (nprobe <= 0: handle default;
+ * ignored by the flat modes).  Replaces IVFLSHIndex::query
+ * lsh_index_optimized.cpp:304 (n_probe_clusters, refine_r).  on_device != 0:
+ * Q, ids and dists are device pointers. */
+int32_t annmi_batch_query_ex(void* h, const float* Q, int32_t B, int32_t k, int32_t rerank, int32_t nprobe,
+                             int64_t* out_ids, float* out_dists, int32_t on_device);
 
 /* Same with device pointers (inputs must be ready when called); synchronous. */
 int32_t annmi_batch_query_device(void* h, const float* dQ, int32_t B, int32_t k, int32_t rerank, int64_t* d_ids,
@@ -90,6 +103,9 @@ int64_t annmi_index_bytes(void* h);
  *        1: SQ per-dim scale (kdim floats)   2: SQ per-dim offset (kdim floats)
  *        3: per-row norm term (n floats)     4: 4 doubles {kdim, row stride
  *           bytes, fp8 scale, max row norm}
+ *        5: IVF list position -> original row id (n uint32)
+ *        6: IVF list offsets (nlist + 1 int64)
+ *        7: IVF centroids (nlist x ceil(d/16)*16 floats)
  * With out == NULL or bytes too small, returns the size without copying;
  * -1 on error. */
 int64_t annmi_export(void* h, int32_t what, void* out, int64_t bytes);

@@ -33,7 +33,9 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <numeric>
 #include <queue>
+#include <random>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -381,6 +383,18 @@ int orc_sq4_nibble(const uint8_t* a, int64_t d) {
     int64_t blk = d / 32, r = d % 32;
     uint8_t byte = a[blk * 16 + (r < 16 ? r : r - 16)];
     return r < 16 ? (byte & 0x0F) : (byte >> 4);
+}
+
+// competitors/ivf_lsh/src/lsh_index_optimized.cpp:553-561: initial centroid
+// rows = shuffle(iota(n), mt19937(seed))[c % n] (libstdc++ std::shuffle, as the
+// reference build uses).  Also returns rand_indices for the empty-cluster rule.
+void orc_kmeans_init(int64_t n, int nlist, uint32_t seed, int64_t* init_rows, int64_t* rand_indices) {
+    std::vector<int> r((size_t)n);
+    std::iota(r.begin(), r.end(), 0);
+    std::shuffle(r.begin(), r.end(), std::mt19937(seed));
+    for (int c = 0; c < nlist; ++c) init_rows[c] = r[(size_t)(c % n)];
+    if (rand_indices)
+        for (int64_t i = 0; i < n; ++i) rand_indices[i] = r[(size_t)i];
 }
 
 }  // extern "C"

@@ -93,6 +93,8 @@ def lib():
         L.orc_sq4_encode.argtypes = [_f32p, _i64, _i64, _f32p, _f32p, _u8p]
         L.orc_sq4_l2.restype = ctypes.c_float
         L.orc_sq4_l2.argtypes = [_u8p, _f32p, _f32p, _i64]
+        L.orc_kmeans_init.restype = None
+        L.orc_kmeans_init.argtypes = [_i64, _i32, ctypes.c_uint32, _i64p, _i64p]
         L.orc_sq4_nibble.restype = _i32
         L.orc_sq4_nibble.argtypes = [_u8p, _i64]
         _lib = L
@@ -244,6 +246,35 @@ def sq4_unpack(codes, dim):
     out.reshape(n, -1, 32)[:, :, :16] = blocks & 0x0F
     out.reshape(n, -1, 32)[:, :, 16:] = blocks >> 4
     return out[:, :dim]
+
+
+# ------------------------------------------------------------------ IVF ----
+def kmeans_init(n, nlist, seed=1337):
+    """(init rows [nlist], rand_indices [n]) per lsh_index_optimized.cpp:553-561."""
+    init = np.empty(nlist, dtype=np.int64)
+    ri = np.empty(n, dtype=np.int64)
+    lib().orc_kmeans_init(int(n), int(nlist), int(seed), init, ri)
+    return init, ri
+
+
+def ivf_probe_exact(centroids, Q, nprobe):
+    """nprobe nearest centroids by (fp64 distance, id)."""
+    ids, _ = exact_topk_f64(centroids, Q, nprobe)
+    return ids
+
+
+def ivf_search_exact(base, perm, list_off, centroids, Q, k, nprobe):
+    """Exact top-k (fp64, lower id on ties) over the union of the nprobe
+    nearest lists: the result an IVF list scan with a complete re-score must
+    return (lsh_index_optimized.cpp:325-333 probe, full-list candidates)."""
+    probes = ivf_probe_exact(centroids, Q, nprobe)
+    out_ids, out_d = [], []
+    for qi, q in enumerate(np.atleast_2d(Q)):
+        rows = np.concatenate([perm[list_off[l]:list_off[l + 1]] for l in probes[qi]]).astype(np.int64)
+        i, dd = exact_topk_subset_f64(base, rows, q, k)
+        out_ids.append(i)
+        out_d.append(dd)
+    return out_ids, out_d
 
 
 # ---------------------------------------------------------------- recall ---
