@@ -25,8 +25,8 @@ MODES = {"exact": ANNMI_EXACT_F32, "fp32": ANNMI_EXACT_F32, "bf16": ANNMI_BF16, 
 # every function declared in include/annmi.h
 EXPORTS = [
     "annmi_default_params", "annmi_create", "annmi_query", "annmi_batch_query", "annmi_batch_query_device",
-    "annmi_batch_query_ex",
-    "annmi_ndist", "annmi_last_batch_stats", "annmi_free", "annmi_last_error", "annmi_index_bytes",
+    "annmi_batch_query_ex", "annmi_query_linear_index",
+    "annmi_ndist", "annmi_last_batch_stats", "annmi_free", "annmi_clone", "annmi_last_error", "annmi_index_bytes",
     "annmi_profile", "annmi_profile_kernels", "annmi_export",
     "gf_create", "gf_query", "gf_ndist", "gf_free", "gf_last_error",
 ]
@@ -82,6 +82,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.annmi_batch_query.argtypes = [vp, fp, i32, i32, i32, i64p, fp]
     L.annmi_batch_query_ex.restype = i32
     L.annmi_batch_query_ex.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp, i32]
+    L.annmi_query_linear_index.restype = i32
+    L.annmi_query_linear_index.argtypes = [vp, vp, i32, i32, vp, vp]
     L.annmi_batch_query_device.restype = i32
     L.annmi_batch_query_device.argtypes = [vp, vp, i32, i32, i32, vp, vp]
     L.annmi_ndist.restype = i64
@@ -90,6 +92,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.annmi_last_batch_stats.argtypes = [vp, u32p, u32p, i32]
     L.annmi_free.restype = None
     L.annmi_free.argtypes = [vp]
+    L.annmi_clone.restype = vp
+    L.annmi_clone.argtypes = [vp]
     L.annmi_last_error.restype = ctypes.c_char_p
     L.annmi_last_error.argtypes = []
     L.annmi_index_bytes.restype = i64

@@ -7,7 +7,7 @@ competitors/gpu_filter/algorithm.py:30-59): ``fit(train, **index_params)``,
 ``batch_query(Q, k, **query_params) -> (ids int64[B, k], dists float32[B, k])``.
 
 Selected in scenarios.yaml with ``backend: mi355x`` and ``mode`` in
-{exact, bf16, fp8, sq8, sq4, ivf}; ``rerank`` (aliases ``r`` as in
+{exact, linear_index, bf16, fp8, sq8, sq4, ivf}; ``rerank`` (aliases ``r`` as in
 competitors/gpu_filter/scenarios.yaml, ``refine_r`` as in the ivf_lsh fork)
 is the over-fetch for the quantized modes; IVF takes ``nlist`` (alias
 ``n_clusters``) at fit and ``nprobe`` (alias ``n_probe_clusters``) at query,
@@ -33,6 +33,7 @@ class Algorithm:
         self._index = None
         self._rerank = 0
         self._nprobe = 0
+        self._compat = False
 
     def fit(self, train: np.ndarray, **index_params) -> None:
         ds_size = index_params.pop("ds_size", None)
@@ -41,8 +42,12 @@ class Algorithm:
         self._backend = str(index_params.pop("backend", "mi355x")).lower()
         if self._backend not in BACKENDS:
             raise ValueError(f"backend {self._backend!r} is not served by this plugin ({BACKENDS})")
-        # 'linear' keeps the reference's exact-scan meaning
+        # mode 'linear_index': the reference's LinearIndex.query semantics bit
+        # for bit (early-abandon values beyond rank 64 included)
         mode = str(index_params.pop("mode", "exact")).lower()
+        self._compat = mode == "linear_index"
+        if self._compat:
+            mode = "exact"
         self._rerank = int(index_params.pop("rerank", index_params.pop("r", index_params.pop("refine_r", 0))) or 0)
         sq_variant = int(index_params.pop("sq_variant", 0))
         max_batch = int(index_params.pop("max_batch", 4096))
@@ -66,6 +71,8 @@ class Algorithm:
         q = np.asarray(query, dtype=np.float32)
         if q.ndim != 1 or q.shape[0] != self._index.d:
             raise RuntimeError(f"query must be a 1-D vector of length {self._index.d}")
+        if self._compat:
+            return self._index.search_linear_index(q[None, :], int(k))[0][0]
         r, npb = self._qp(query_params)
         ids, _ = self._index.search(q[None, :], int(k), rerank=r, nprobe=npb)
         return ids[0]
@@ -77,6 +84,10 @@ class Algorithm:
 
     def batch_query(self, queries, k: int, **query_params):
         self._need_index()
+        if self._compat:
+            if _is_device(queries):
+                queries = queries.cpu().numpy()
+            return self._index.search_linear_index(np.asarray(queries, dtype=np.float32), int(k))
         r, npb = self._qp(query_params)
         if _is_device(queries):
             return self._index.search_device(queries, int(k), rerank=r, nprobe=npb)

@@ -43,6 +43,9 @@ hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st);
 hipError_t launch_prep_queries(int kind, const float* Q, int B, int d, int kdim, void* Qscan, float* qn, float* qscale,
                                float* margin, float maxnorm, float alpha_x, const float* scale, const float* offset,
                                hipStream_t st);
+hipError_t launch_linear_index_query(const float* X, int64_t xstride, int64_t n, int d, const float* dq, int k,
+                                     float* P, float* BT, float* R, float* bkey, uint32_t* bid, int64_t* d_ids,
+                                     float* d_vals, uint32_t* d_flag, hipStream_t st);
 
 namespace {
 
@@ -174,7 +177,22 @@ struct Handle {
     uint32_t* d_gq = nullptr;
     int64_t gq_cap = 0;
     uint32_t* d_next = nullptr;
+    int next_cap = 0;
     std::vector<int64_t> last_scanned;
+    // a clone shares every index array with its source and owns only its
+    // stream and workspace
+    bool is_clone = false;
+    // linear_index-compatible query workspace
+    float* li_P = nullptr;
+    float* li_BT = nullptr;
+    float* li_R = nullptr;
+    float* li_bkey = nullptr;
+    uint32_t* li_bid = nullptr;
+    float* li_q = nullptr;
+    int64_t* li_ids = nullptr;
+    float* li_vals = nullptr;
+    uint32_t* li_flag = nullptr;
+    int li_k = 0;
 };
 
 void free_ws(Handle* h) {
@@ -205,15 +223,26 @@ void free_handle(Handle* h) {
     if (!h) return;
     if (h->st) (void)hipStreamSynchronize(h->st);
     free_ws(h);
-    if (h->own_x32) dfree(h->X32);
-    dfree(h->xn32);
-    if (h->own_xs) dfree(h->Xs);
-    if (h->xn != h->xn32) dfree(h->xn);
-    dfree(h->sq_scale);
-    dfree(h->sq_offset);
+    if (!h->is_clone) {
+        if (h->own_x32) dfree(h->X32);
+        dfree(h->xn32);
+        if (h->own_xs) dfree(h->Xs);
+        if (h->xn != h->xn32) dfree(h->xn);
+        dfree(h->sq_scale);
+        dfree(h->sq_offset);
+        dfree(h->perm);
+        dfree(h->centroids);
+    }
     dfree(h->scratch_keys);
-    dfree(h->perm);
-    dfree(h->centroids);
+    dfree(h->li_P);
+    dfree(h->li_BT);
+    dfree(h->li_R);
+    dfree(h->li_bkey);
+    dfree(h->li_bid);
+    dfree(h->li_q);
+    dfree(h->li_ids);
+    dfree(h->li_vals);
+    dfree(h->li_flag);
     dfree(h->d_probe);
     dfree(h->d_probe_d);
     dfree(h->d_items);
@@ -672,6 +701,114 @@ int32_t batch_impl(Handle* h, const float* Q, int32_t B, int32_t k, int32_t R, i
     return kk;
 }
 
+// src/linear_index.cpp:90-121 bit for bit (compat.hip), one query at a time.
+int32_t linear_index_impl(Handle* h, const float* Q, int32_t B, int32_t k, int64_t* ids, float* vals) {
+    g_err[0] = 0;
+    if (!h || B < 0 || k <= 0 || (B > 0 && (!Q || !ids || !vals))) {
+        set_err("bad arguments");
+        return -1;
+    }
+    if (h->kind != SCAN_F32 || h->ivf) {
+        set_err("the linear_index query needs an exact (fp32) index");
+        return -1;
+    }
+    if (k > 2048) {
+        set_err("k=%d exceeds 2048", k);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    const int64_t n = h->n;
+    const int d = h->d;
+    const int W = d / 96 + 2;
+    const int64_t G = (n + 4095) / 4096;
+    if (!h->li_P) {
+        HIP_OR(hipMalloc(&h->li_P, (size_t)n * W * 4), -1);
+        HIP_OR(hipMalloc(&h->li_BT, (size_t)G * 64 * 4), -1);
+        HIP_OR(hipMalloc(&h->li_R, (size_t)G * 64 * 4), -1);
+        HIP_OR(hipMalloc(&h->li_q, (size_t)d * 4), -1);
+        HIP_OR(hipMalloc(&h->li_flag, 4), -1);
+    }
+    if (h->li_k < k) {
+        dfree(h->li_bkey);
+        dfree(h->li_bid);
+        dfree(h->li_ids);
+        dfree(h->li_vals);
+        h->li_bkey = nullptr;
+        h->li_bid = nullptr;
+        h->li_ids = nullptr;
+        h->li_vals = nullptr;
+        h->li_k = k;
+        HIP_OR(hipMalloc(&h->li_bkey, (size_t)G * k * 4), -1);
+        HIP_OR(hipMalloc(&h->li_bid, (size_t)G * k * 4), -1);
+        HIP_OR(hipMalloc(&h->li_ids, (size_t)k * 8), -1);
+        HIP_OR(hipMalloc(&h->li_vals, (size_t)k * 4), -1);
+    }
+    const int kk = (int)std::min<int64_t>(k, n);
+    hipStream_t st = h->st;
+    for (int b = 0; b < B; ++b) {
+        uint32_t zero = 0, flag = 0;
+        HIP_OR(hipMemcpyAsync(h->li_q, Q + (int64_t)b * d, (size_t)d * 4, hipMemcpyHostToDevice, st), -1);
+        HIP_OR(hipMemcpyAsync(h->li_flag, &zero, 4, hipMemcpyHostToDevice, st), -1);
+        HIP_OR(launch_linear_index_query(h->X32, h->kdim32, n, d, h->li_q, k, h->li_P, h->li_BT, h->li_R, h->li_bkey,
+                                         h->li_bid, h->li_ids, h->li_vals, h->li_flag, st),
+               -1);
+        HIP_OR(hipMemcpyAsync(ids + (int64_t)b * k, h->li_ids, (size_t)kk * 8, hipMemcpyDeviceToHost, st), -1);
+        HIP_OR(hipMemcpyAsync(vals + (int64_t)b * k, h->li_vals, (size_t)kk * 4, hipMemcpyDeviceToHost, st), -1);
+        HIP_OR(hipMemcpyAsync(&flag, h->li_flag, 4, hipMemcpyDeviceToHost, st), -1);
+        HIP_OR(hipStreamSynchronize(st), -1);
+        if (flag) {
+            set_err("query %d: more than 2048 rows tie at the rank-%d value", b, kk);
+            return -1;
+        }
+        for (int j = kk; j < k; ++j) {
+            ids[(int64_t)b * k + j] = -1;
+            vals[(int64_t)b * k + j] = __builtin_inff();
+        }
+        h->ndist += n;  // linear_index.cpp:108
+    }
+    return kk;
+}
+
+// Share an index: same device arrays, new stream and workspace.
+Handle* clone_impl(Handle* src) {
+    Handle* h = new Handle();
+    h->p = src->p;
+    h->n = src->n;
+    h->d = src->d;
+    h->kdim32 = src->kdim32;
+    h->X32 = src->X32;
+    h->xn32 = src->xn32;
+    h->maxnorm = src->maxnorm;
+    h->kind = src->kind;
+    h->kdim = src->kdim;
+    h->Xs = src->Xs;
+    h->xs_stride = src->xs_stride;
+    h->xn = src->xn;
+    h->alpha_x = src->alpha_x;
+    h->sq_scale = src->sq_scale;
+    h->sq_offset = src->sq_offset;
+    h->cap = src->cap;
+    h->ivf = src->ivf;
+    h->nlist = src->nlist;
+    h->perm = src->perm;
+    h->list_off = src->list_off;
+    h->centroids = src->centroids;
+    h->is_clone = true;
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        set_err("hipStreamCreate failed");
+        delete h;
+        return nullptr;
+    }
+    if (src->cent) {
+        h->cent = clone_impl(src->cent);
+        if (!h->cent) {
+            free_handle(h);
+            return nullptr;
+        }
+    }
+    return h;
+}
+
 // ------------------------------------------------------------------ IVF --
 // k-means (lsh_index_optimized.cpp:551-616) on the GPU; see ivf.hip.
 bool ivf_build(Handle* h) {
@@ -846,37 +983,64 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
             qlist[(size_t)l].push_back((uint32_t)q);
             scanned[(size_t)q] += h->list_off[(size_t)l + 1] - h->list_off[(size_t)l];
         }
-    const int64_t S = std::max<int64_t>(256, (4 * (int64_t)rsel + nprobe - 1) / nprobe);
+    // geometric passes over every probed list: pass 0 takes the first S rows
+    // of each list with an open filter, pass p the rows [S*8^(p-1), S*8^p);
+    // before each later pass the filter becomes the rsel-th smallest key seen
+    // so far, so each pass adds about rsel * 7 candidates whatever the list
+    // lengths (hub lists of millions of rows included).
+    int64_t S = std::max<int64_t>(256, (4 * (int64_t)rsel + nprobe - 1) / nprobe);
+    S = std::max<int64_t>(16, std::min<int64_t>(S, h->cap / (2 * (int64_t)nprobe)));  // pass 0 fits the lists
     const uint32_t CH = 256;
     std::vector<uint32_t> gq;
-    std::vector<IvfItem> itA, itB;
-    double bytesA = 0, bytesB = 0, flopsA = 0, flopsB = 0;
+    std::vector<std::vector<IvfItem>> passes;
+    std::vector<double> pbytes, pflops;
+    auto pass_bounds = [&](int p) -> std::pair<int64_t, int64_t> {
+        int64_t lo = p == 0 ? 0 : S, hi = S;
+        for (int i = 0; i < p; ++i) {
+            lo = hi;
+            hi = hi * 8;
+        }
+        if (p == 0) lo = 0;
+        return {lo, hi};
+    };
     for (int l = 0; l < h->nlist; ++l) {
         const auto& ql = qlist[(size_t)l];
         if (ql.empty()) continue;
-        const uint32_t b0 = (uint32_t)h->list_off[(size_t)l], b1 = (uint32_t)h->list_off[(size_t)l + 1];
-        const uint32_t sA = (uint32_t)std::min<int64_t>(b1, b0 + S);
+        const int64_t b0 = h->list_off[(size_t)l], b1 = h->list_off[(size_t)l + 1];
         for (size_t g0 = 0; g0 < ql.size(); g0 += 16) {
             const uint32_t nq = (uint32_t)std::min<size_t>(16, ql.size() - g0);
             const uint32_t qoff = (uint32_t)gq.size();
             for (uint32_t t = 0; t < nq; ++t) gq.push_back(ql[g0 + t]);
-            for (uint32_t r = b0; r < sA; r += CH) {
-                itA.push_back(IvfItem{r, std::min(sA, r + CH), qoff, nq});
-                bytesA += (double)(std::min(sA, r + CH) - r) * h->xs_stride;
-                flopsA += 2.0 * (std::min(sA, r + CH) - r) * nq * h->d;
-            }
-            for (uint32_t r = sA; r < b1; r += CH) {
-                itB.push_back(IvfItem{r, std::min(b1, r + CH), qoff, nq});
-                bytesB += (double)(std::min(b1, r + CH) - r) * h->xs_stride;
-                flopsB += 2.0 * (std::min(b1, r + CH) - r) * nq * h->d;
+            for (int p = 0;; ++p) {
+                auto [lo, hi] = pass_bounds(p);
+                if (b0 + lo >= b1) break;
+                if ((int)passes.size() <= p) {
+                    passes.emplace_back();
+                    pbytes.push_back(0);
+                    pflops.push_back(0);
+                }
+                const int64_t e = std::min<int64_t>(b1, b0 + hi);
+                for (int64_t r = b0 + lo; r < e; r += CH) {
+                    const uint32_t re = (uint32_t)std::min<int64_t>(e, r + CH);
+                    passes[(size_t)p].push_back(IvfItem{(uint32_t)r, re, qoff, nq});
+                    pbytes[(size_t)p] += (double)(re - r) * h->xs_stride;
+                    pflops[(size_t)p] += 2.0 * (re - r) * nq * h->d;
+                }
             }
         }
     }
-    const int nA = (int)itA.size(), nB = (int)itB.size();
-    if (h->items_cap < nA + nB) {
+    std::vector<IvfItem> all;
+    std::vector<int> pstart;
+    for (auto& v : passes) {
+        pstart.push_back((int)all.size());
+        all.insert(all.end(), v.begin(), v.end());
+    }
+    pstart.push_back((int)all.size());
+    const int npass = (int)passes.size();
+    if (h->items_cap < (int)all.size()) {
         dfree(h->d_items);
         h->d_items = nullptr;
-        h->items_cap = std::max(nA + nB, 1024);
+        h->items_cap = std::max((int)all.size(), 1024);
         HIP_OR(hipMalloc(&h->d_items, (size_t)h->items_cap * sizeof(IvfItem)), false);
     }
     if (h->gq_cap < (int64_t)gq.size() || !h->d_gq) {
@@ -885,15 +1049,19 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
         h->gq_cap = std::max<int64_t>((int64_t)gq.size(), 4096);
         HIP_OR(hipMalloc(&h->d_gq, (size_t)h->gq_cap * 4), false);
     }
-    if (!h->d_next) HIP_OR(hipMalloc(&h->d_next, 2 * sizeof(uint32_t)), false);
-    itA.insert(itA.end(), itB.begin(), itB.end());
-    if (!itA.empty())
-        HIP_OR(hipMemcpyAsync(h->d_items, itA.data(), itA.size() * sizeof(IvfItem), hipMemcpyHostToDevice, st), false);
+    if (!h->d_next || h->next_cap < npass) {
+        dfree(h->d_next);
+        h->d_next = nullptr;
+        h->next_cap = std::max(npass, 16);
+        HIP_OR(hipMalloc(&h->d_next, (size_t)h->next_cap * sizeof(uint32_t)), false);
+    }
+    if (!all.empty())
+        HIP_OR(hipMemcpyAsync(h->d_items, all.data(), all.size() * sizeof(IvfItem), hipMemcpyHostToDevice, st), false);
     if (!gq.empty()) HIP_OR(hipMemcpyAsync(h->d_gq, gq.data(), gq.size() * 4, hipMemcpyHostToDevice, st), false);
     HIP_OR(launch_fill_u32(h->cnt, 0u, B, st), false);
     HIP_OR(launch_fill_u32(h->ovf, 0u, B, st), false);
     HIP_OR(launch_fill_f32(h->thr, __builtin_inff(), B, st), false);
-    HIP_OR(launch_fill_u32(h->d_next, 0u, 2, st), false);
+    HIP_OR(launch_fill_u32(h->d_next, 0u, h->next_cap, st), false);
     CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
     pr.ev_prep1 = pr.ev();
     IvfScanArgs ia{};
@@ -908,25 +1076,25 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
     ia.thr = h->thr;
     ia.gq = h->d_gq;
     ia.cand = cb;
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1) {
+    for (int pass = 0; pass < npass; ++pass) {
+        if (pass > 0) {
             SelectArgs sa{cb, B, rsel, h->slack, h->thr, h->ovf};
             int e0 = pr.ev();
             HIP_OR(launch_select_threshold(sa, st), false);
             int e1 = pr.ev();
             sel_ev.push_back({e0, e1});
         }
-        ia.items = h->d_items + (pass == 0 ? 0 : nA);
-        ia.nitems = pass == 0 ? nA : nB;
+        ia.items = h->d_items + pstart[(size_t)pass];
+        ia.nitems = pstart[(size_t)pass + 1] - pstart[(size_t)pass];
         ia.next = h->d_next + pass;
         int e0 = pr.ev();
         HIP_OR(launch_ivf_scan(ia, st), false);
         int e1 = pr.ev();
         scan_ev.push_back({e0, e1});
         scan_rows.push_back(0.0);
-        scan_names.push_back(pass == 0 ? "ivf_scan_kernel<sample>" : "ivf_scan_kernel<lists>");
-        pr.bytes_override.push_back(pass == 0 ? bytesA : bytesB);
-        pr.flops_override.push_back(pass == 0 ? flopsA : flopsB);
+        scan_names.push_back("ivf_scan_kernel");
+        pr.bytes_override.push_back(pbytes[(size_t)pass]);
+        pr.flops_override.push_back(pflops[(size_t)pass]);
     }
     if (!finish_step(h, B, k, kk, rsel, ids_out, d_out, cb, false, pr)) return false;
     h->last_scanned.insert(h->last_scanned.end(), scanned.begin(), scanned.end());
@@ -992,6 +1160,21 @@ int32_t annmi_last_batch_stats(void* hv, uint32_t* n_rescored, uint32_t* flags, 
 }
 
 void annmi_free(void* h) { free_handle(static_cast<Handle*>(h)); }
+
+int32_t annmi_query_linear_index(void* h, const float* Q, int32_t B, int32_t k, int64_t* out_ids,
+                                 float* out_values) {
+    return linear_index_impl(static_cast<Handle*>(h), Q, B, k, out_ids, out_values);
+}
+
+void* annmi_clone(void* h) {
+    if (!h) {
+        set_err("null handle");
+        return nullptr;
+    }
+    Handle* src = static_cast<Handle*>(h);
+    std::lock_guard<std::mutex> lk(src->mu);
+    return clone_impl(src);
+}
 
 const char* annmi_last_error(void) { return g_err; }
 

@@ -60,6 +60,19 @@ class Index:
         self._h = ctypes.c_void_p(h)
         self._L = L
 
+    def clone(self) -> "Index":
+        """Another handle on the same device-resident index with its own HIP
+        stream and workspace (concurrent batches).  Keeps this index alive."""
+        h = self._L.annmi_clone(self._h)
+        if not h:
+            raise RuntimeError(_lib.last_error())
+        c = Index.__new__(Index)
+        c.mode, c.n, c.d = self.mode, self.n, self.d
+        c._keep = self
+        c._h = ctypes.c_void_p(h)
+        c._L = self._L
+        return c
+
     # ---------------------------------------------------------------- queries
     def _check_q(self, Q):
         if Q.ndim != 2 or Q.shape[1] != self.d:
@@ -80,6 +93,21 @@ class Index:
         if got < 0:
             raise RuntimeError(_lib.last_error())
         return ids[:, :got], dd[:, :got]
+
+    def search_linear_index(self, Q, k: int):
+        """The reference's LinearIndex.query semantics bit for bit (ids and
+        the fp32 values it ranks, early-abandon partial sums included)."""
+        Q = np.ascontiguousarray(np.atleast_2d(np.asarray(Q, dtype=np.float32)))
+        self._check_q(Q)
+        B = Q.shape[0]
+        ids = np.empty((B, k), dtype=np.int64)
+        vals = np.empty((B, k), dtype=np.float32)
+        got = self._L.annmi_query_linear_index(self._h, Q.ctypes.data_as(ctypes.c_void_p), B, int(k),
+                                               ids.ctypes.data_as(ctypes.c_void_p),
+                                               vals.ctypes.data_as(ctypes.c_void_p))
+        if got < 0:
+            raise RuntimeError(_lib.last_error())
+        return ids[:, :got], vals[:, :got]
 
     def search_device(self, Q, k: int, rerank: int = 0, nprobe: int = 0):
         """Same for a torch device tensor; returns device tensors."""

@@ -123,6 +123,55 @@ def cpu_baseline(base_t, Q_t, budget_s=15.0, max_q=200):
     }
 
 
+def bench_ivf(args):
+    """Served IVF + SQ8: 10M x 768 Zipf(1.1) mixture,
+    k-means nlist = 4096, nprobe in {16, 64}, Poisson arrivals, request sizes
+    lognormal (median 8, sigma 1, clipped to [1, 512]), continuous batching on
+    4 handles sharing the index.  Prints one JSON line."""
+    import torch
+
+    from annvedi_amd import Index, data
+    from annvedi_amd.serving import Server, lognormal_sizes, run_open_loop
+
+    dev = "cuda:0"
+    n, d, k = args.n, 768, args.k
+    log(f"[ivf] generating {n} x {d} Zipf mixture in HBM")
+    base = data.mixture_torch(n, d, 4096, 0.3, 2004, center_seed=1004, normalize=False, zipf_s=1.1, device=dev)
+    sizes = lognormal_sizes(args.requests, seed=4002)
+    Qd = data.mixture_torch(int(sizes.sum()) + 1024, d, 4096, 0.3, 3004, center_seed=1004, normalize=False,
+                            zipf_s=1.1, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    idx = Index(base, mode="ivf", nlist=4096, rerank=args.rerank or 400, max_batch=512)
+    build_s = time.perf_counter() - t0
+    log(f"[ivf] build {build_s:.1f}s, index bytes {idx.nbytes / 1e9:.2f} GB")
+    exact = Index(base, mode="exact")
+    Qcal = Qd[-1024:]
+    gt, _ = exact.search_device(Qcal, k)
+    Qh = Qd[:-1024].cpu().numpy()
+    runs = []
+    for nprobe in (16, 64):
+        ids, _ = idx.search_device(Qcal, k, nprobe=nprobe)
+        rc = float(recall_vs_exact(base, Qcal, ids, gt, k).mean())
+        for rate in args.rates:
+            with Server(idx, workers=4, max_batch=512, nprobe=nprobe) as srv:
+                run_open_loop(srv, Qh, sizes[:50], qps=rate, k=k)  # warm the handles
+                r = run_open_loop(srv, Qh, sizes, qps=rate, k=k)
+            runs.append({"nprobe": nprobe, "offered_qps": rate, "qps": round(r["qps"], 1),
+                         "p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3), "recall_at_100": rc})
+            log(f"[ivf] {runs[-1]}")
+    best = max((x for x in runs if x["nprobe"] == 16), key=lambda x: x["qps"])
+    res = {"metric": "served IVF+SQ8 QPS (10Mx768, nlist 4096, Poisson, continuous batching); p50/p99 ms",
+           "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": len(runs), "warmup": 1,
+           "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8",
+           "data": "synthetic",
+           "config": {"workload": "ivf-10m-768", "n": n, "d": d, "nlist": 4096, "zipf_s": 1.1, "k": k,
+                      "request_sizes": "lognormal median 8 sigma 1 [1,512]", "requests": int(len(sizes)),
+                      "workers": 4, "slot": 512, "rerank": args.rerank or 400},
+           "build_s": round(build_s, 2), "runs": runs}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,12 +184,18 @@ def main():
     ap.add_argument("--rerank", type=int, default=0, help="fixed R (0: calibrate to recall >= 0.99)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain fp32 path")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="mixed-10m-384", choices=["mixed-10m-384", "ivf-10m-768"])
+    ap.add_argument("--requests", type=int, default=1500, help="served IVF: requests per rate")
+    ap.add_argument("--rates", type=float, nargs="*", default=[2000, 10000, 50000])
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(the index would shard by rows or queries; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "ivf-10m-768":
+        return bench_ivf(args)
 
     import torch
 

@@ -72,6 +72,14 @@ int32_t annmi_batch_query(void* h, const float* Q, int32_t B, int32_t k, int32_t
 int32_t annmi_batch_query_ex(void* h, const float* Q, int32_t B, int32_t k, int32_t rerank, int32_t nprobe,
                              int64_t* out_ids, float* out_dists, int32_t on_device);
 
+/* The reference's exact scan reproduced bit for bit, B host queries one at a
+ * time: ids AND the fp32 values it ranks, including its early-abandon
+ * partial sums beyond rank 64 (64-slot threshold, checkpoints every 96
+ * dims).  Replaces LinearIndex::query src/linear_index.cpp:90-121 with
+ * linear_worker :27-60 and compute_l2_distance src/simd.h:32-64 (AVX-512
+ * branch).  Needs an ANNMI_EXACT_F32 handle.  Returns min(k, n) or -1. */
+int32_t annmi_query_linear_index(void* h, const float* Q, int32_t B, int32_t k, int64_t* out_ids, float* out_values);
+
 /* Same with device pointers (inputs must be ready when called); synchronous. */
 int32_t annmi_batch_query_device(void* h, const float* dQ, int32_t B, int32_t k, int32_t rerank, int64_t* d_ids,
                                  float* d_dists);
@@ -87,6 +95,12 @@ int32_t annmi_last_batch_stats(void* h, uint32_t* n_rescored, uint32_t* flags, i
 
 /* Release the index.  NULL-safe.  Replaces gf_free gpu_filter.cu:180-190. */
 void annmi_free(void* h);
+
+/* A second handle on the same device-resident index (no copy) with its own
+ * HIP stream and workspace, so batches on different handles run
+ * concurrently on the device.  The source must outlive its clones.  No
+ * reference counterpart (the reference serves one query at a time). */
+void* annmi_clone(void* h);
 
 /* Thread-local message of the last failure on this thread.
  * Replaces gf_last_error gpu_filter.cu:77 (process-global there). */

@@ -1,0 +1,35 @@
+"""Continuous batching on the GPU: every answer equals the same request
+served alone, and batches on different handles overlap in time."""
+import numpy as np
+import pytest
+
+from annvedi_amd import Index, data
+from annvedi_amd.serving import Server, lognormal_sizes, run_open_loop
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mode", ["exact", "bf16", "ivf"])
+def test_served_answers_equal_alone_answers(mode):
+    base, Q = data.workload("exact-1m-384", n=200_000, n_queries=3000)
+    kw = dict(nlist=128, kmeans_iters=5) if mode == "ivf" else {}
+    idx = Index(base, mode=mode, rerank=300, **kw)
+    sizes = lognormal_sizes(120, seed=11)
+    sizes = sizes[np.cumsum(sizes) <= Q.shape[0]]
+    with Server(idx, workers=3, max_batch=512, rerank=300, nprobe=16) as srv:
+        res = run_open_loop(srv, Q, sizes, qps=30000, k=100)
+        batches = list(srv.batches)
+    off = 0
+    for b, (ids, dd) in zip(sizes, res["answers"]):
+        a_ids, a_dd = idx.search(Q[off:off + b], 100, rerank=300, nprobe=16)
+        np.testing.assert_array_equal(ids, a_ids)
+        np.testing.assert_array_equal(dd, a_dd)
+        off += b
+    # requests that arrived while others ran shared a batch, and batches on
+    # different workers overlapped in time
+    assert max(nr for *_, nr in batches) > 1
+    overl = sum(1 for i, a in enumerate(batches) for b in batches[i + 1:]
+                if a[0] != b[0] and a[2] < b[3] and b[2] < a[3])
+    print(f"{mode}: {len(batches)} batches, {overl} overlapping pairs, QPS {res['qps']:.0f}, "
+          f"p50 {res['p50_ms']:.2f} ms, p99 {res['p99_ms']:.2f} ms")
+    assert overl > 0
