@@ -10,7 +10,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libannmi.so")
+# ANNMI_LIB: another build of the same library (A/B timing of kernel variants)
+LIB_PATH = os.environ.get("ANNMI_LIB") or os.path.join(HERE, "libannmi.so")
 
 ANNMI_EXACT_F32 = 0
 ANNMI_BF16 = 1

@@ -27,6 +27,7 @@
 #include "common.h"
 
 #include <cstdio>
+#include <cstdlib>
 
 namespace {
 
@@ -37,6 +38,7 @@ struct PolF32 {
     static constexpr int kDimsPerLoad = 4;     // per lane per 16B load
     static constexpr int kChunk = 16;          // dims per wave-wide load step (4 lane groups)
     static constexpr int kElemBytes = 4;
+    static constexpr int kRing = 2;       // chunks of row fragments in flight per wave (wide scan)
     typedef v4f Acc;
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         f32x4 af = __builtin_bit_cast(f32x4, a);
@@ -52,6 +54,7 @@ struct PolBF16 {
     static constexpr int kDimsPerLoad = 8;
     static constexpr int kChunk = 32;
     static constexpr int kElemBytes = 2;
+    static constexpr int kRing = 4;       // chunks of row fragments in flight per wave (wide scan)
     typedef v4f Acc;
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
@@ -63,6 +66,7 @@ struct PolFP8 {
     static constexpr int kDimsPerLoad = 16;
     static constexpr int kChunk = 64;
     static constexpr int kElemBytes = 1;
+    static constexpr int kRing = 4;       // chunks of row fragments in flight per wave (wide scan)
     typedef v4f Acc;
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         long a0 = ((long)(uint32_t)a[1] << 32) | (uint32_t)a[0];
@@ -79,6 +83,7 @@ struct PolSQ8 {
     static constexpr int kDimsPerLoad = 16;
     static constexpr int kChunk = 64;
     static constexpr int kElemBytes = 1;
+    static constexpr int kRing = 4;       // chunks of row fragments in flight per wave (wide scan)
     typedef i32x4 Acc;
     __device__ static inline void mma(i32x4& acc, const i32x4& a, const i32x4& b) {
         // codes are uint8 in [0,255]; the MFMA reads signed int8, so flip the
@@ -95,6 +100,7 @@ struct PolSQ4 {
     static constexpr int kDimsPerLoad = 16;
     static constexpr int kChunk = 64;
     static constexpr int kElemBytes = 0;  // packed: 2 dims per byte
+    static constexpr int kRing = 8;       // chunks of row fragments in flight per wave (wide scan)
     typedef i32x4 Acc;
     __device__ static inline void mma(i32x4& acc, const i32x4& a, const i32x4& b) {
         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
@@ -102,22 +108,51 @@ struct PolSQ4 {
 };
 
 
-// Row fragment load: 16 bytes of row `row` covering dims [dim0, dim0 + kDimsPerLoad).
+// Row fragments.  A lane's bytes of one row chunk are loaded raw (16 bytes;
+// SQ4: 8 packed bytes) and turned into the MFMA B operand where they are
+// used, so a prefetched load is not waited for before its MFMA.
 template <class P>
-__device__ inline i32x4 load_row_frag(const uint8_t* X, int64_t xstride, int64_t row, int dim0) {
+struct RawOf {
+    typedef i32x4 T;
+};
+template <>
+struct RawOf<PolSQ4> {
+    typedef uint2 T;
+};
+
+// bytes of one row chunk, and of one lane group's share of it
+template <class P>
+__host__ __device__ constexpr int chunk_bytes() {
+    return P::kElemBytes == 0 ? P::kChunk / 2 : P::kChunk * P::kElemBytes;
+}
+template <class P>
+__host__ __device__ constexpr int lane_bytes() {
+    return P::kElemBytes == 0 ? P::kDimsPerLoad / 2 : P::kDimsPerLoad * P::kElemBytes;
+}
+
+template <class P>
+__device__ inline typename RawOf<P>::T load_raw(const uint8_t* p) {
+    return *reinterpret_cast<const typename RawOf<P>::T*>(p);
+}
+
+template <class P>
+__device__ inline i32x4 to_frag(const typename RawOf<P>::T& v) {
     if constexpr (P::kElemBytes == 0) {
-        // SQ4: chunk c covers 64 dims = 32 packed bytes; lane group g reads
-        // bytes [32c + 8g, +8): dim0 here is 64c + 16g.
-        const uint8_t* p = X + row * xstride + (dim0 >> 1);
-        uint2 v = *reinterpret_cast<const uint2*>(p);
-        // unpack 8 bytes -> 16 int8: lows then highs
+        // SQ4: chunk c covers 64 dims = 32 packed bytes; lane group g holds
+        // bytes [32c + 8g, +8), unpacked to 16 int8: low nibbles, then high
         uint32_t lo0 = v.x & 0x0F0F0F0Fu, lo1 = v.y & 0x0F0F0F0Fu;
         uint32_t hi0 = (v.x >> 4) & 0x0F0F0F0Fu, hi1 = (v.y >> 4) & 0x0F0F0F0Fu;
         return (i32x4){(int)lo0, (int)lo1, (int)hi0, (int)hi1};
     } else {
-        const uint8_t* p = X + row * xstride + (int64_t)dim0 * P::kElemBytes;
-        return *reinterpret_cast<const i32x4*>(p);
+        return v;
     }
+}
+
+// Fragment of row `row` covering dims [dim0, dim0 + kDimsPerLoad).
+template <class P>
+__device__ inline i32x4 load_row_frag(const uint8_t* X, int64_t xstride, int64_t row, int dim0) {
+    const int64_t off = P::kElemBytes == 0 ? (dim0 >> 1) : (int64_t)dim0 * P::kElemBytes;
+    return to_frag<P>(load_raw<P>(X + row * xstride + off));
 }
 
 // Query tile in LDS: 16*QW queries x kdim elements, stored in the scan
@@ -304,33 +339,102 @@ hipError_t launch_t(const ScanArgs& a, hipStream_t st) {
 
 
 // ------------------------------------------------------------------ wide --
-// Large batches: query-stationary persistent kernel.  A workgroup (4-8 waves)
-// keeps a tile of 16*QS queries in LDS for its whole life and streams a
-// contiguous row range; every wave scores 16*QS queries x 16*RS rows per
-// step, row fragments loaded from L2/HBM straight into MFMA B registers
-// (double-buffered over the K chunks), query fragments from LDS.  Per K
-// chunk a wave issues QS*RS MFMA groups for RS global and QS LDS fragment
-// loads, which keeps LDS traffic at ~half its bandwidth at full MFMA rate.
+// Large batches: query-stationary kernel over short row chunks.  A workgroup
+// (4-8 waves) stages a tile of 16*QS queries in LDS and scores it against
+// one chunk of rows; every wave takes 16*RS rows at a time and multiplies
+// them, fragment by fragment, against all 16*QS queries.  Row fragments go
+// from L2/HBM straight into MFMA B registers through a ring of kRing chunks
+// in flight; query fragments come from LDS one chunk ahead of their MFMAs.
 //
 // Workgroup -> work mapping is XCD-aware: dispatch places workgroup b on XCD
-// b % 8, so the QTILES query tiles of one row range are given consecutive
-// slots b/8 on the same XCD.  They stream the same rows at the same time and
-// share that XCD's L2, so HBM sees each row once per launch.
+// b % 8, and each XCD walks its own contiguous run of row chunks, row chunk
+// major and query tile minor.  The workgroups resident on an XCD at any time
+// therefore cover one or a few row chunks for all query tiles and share
+// those rows through the XCD's L2: HBM sees each row about once per launch.
+//
+// Passing (query, row) pairs are queued per wave in LDS (slot from an LDS
+// atomic) and moved to the per-query lists in bulk, so the scan never waits
+// on a returning global atomic, which would also wait for the row loads in
+// flight.  A full queue spills single entries straight to the lists.
+
+struct WideQueue {
+    uint32_t* row;  // (row - chunk begin) << 7 | query slot in the tile
+    float* key;
+    uint32_t* count;  // LDS counter of this wave
+    int cap;
+};
+
+// Append one pair (lanes diverge: only passing lanes get here).
+__device__ inline void wide_append(const ScanArgs& a, const WideQueue& wq, int qi, int m, uint32_t roff,
+                                   uint32_t row, float dist) {
+    const uint32_t pos = atomicAdd(wq.count, 1u);
+    if (pos < (uint32_t)wq.cap) {
+        wq.row[pos] = (roff << 7) | (uint32_t)m;
+        wq.key[pos] = dist;
+        return;
+    }
+    const uint32_t off = atomicAdd(&a.cand.cnt[qi], 1u);
+    if (off < (uint32_t)a.cand.cap) {
+        const int64_t sl = (int64_t)qi * a.cand.cap + off;
+        a.cand.key[sl] = dist;
+        a.cand.rid[sl] = a.rid_map ? a.rid_map[row] : row;
+    }
+}
+
+// Move the wave's queued pairs to the per-query lists: one returning global
+// atomic per entry, eight per lane issued before a wait.
+__device__ inline void wide_flush(const ScanArgs& a, const WideQueue& wq, int q0, int64_t r_begin, int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t cnt = __builtin_amdgcn_readfirstlane(*wq.count);
+    const int count = cnt < (uint32_t)wq.cap ? (int)cnt : wq.cap;
+    constexpr int U = 8;
+    for (int e0 = 0; e0 < count; e0 += 64 * U) {
+        uint32_t v[U], off[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int e = e0 + 64 * j + lane;
+            v[j] = e < count ? wq.row[e] : 0u;
+            off[j] = e < count ? atomicAdd(&a.cand.cnt[q0 + (int)(v[j] & 127u)], 1u) : ~0u;
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int e = e0 + 64 * j + lane;
+            if (off[j] < (uint32_t)a.cand.cap) {
+                const int qi = q0 + (int)(v[j] & 127u);
+                const uint32_t row = (uint32_t)(r_begin + (v[j] >> 7));
+                const int64_t sl = (int64_t)qi * a.cand.cap + off[j];
+                a.cand.key[sl] = wq.key[e];
+                a.cand.rid[sl] = a.rid_map ? a.rid_map[row] : row;
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) *wq.count = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <class P, int QS, int RS, int kWideWaves>
-__global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, int qtiles, int rpx, int64_t range_len) {
+__global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, int qtiles, int64_t chunk_rows,
+                                                                    int64_t nrc, int qcap) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
     const int n = lane & 15;
+    // work item of this workgroup: XCD x owns row chunks [x*per, (x+1)*per)
     const int xcd = blockIdx.x & 7;
-    const int slot = blockIdx.x >> 3;
-    if (slot >= qtiles * rpx) return;
-    const int qtile = slot % qtiles;
-    const int rsplit = xcd * rpx + slot / qtiles;
-    const int64_t r_begin = a.row_begin + (int64_t)rsplit * range_len;
-    if (r_begin >= a.row_end) return;
-    const int64_t r_end = r_begin + range_len < a.row_end ? r_begin + range_len : a.row_end;
+    const int64_t s = blockIdx.x >> 3;
+    const int64_t per = (nrc + 7) / 8;
+    if (s / qtiles >= per) return;
+    const int64_t rc = xcd * per + s / qtiles;
+    if (rc >= nrc) return;
+    const int qtile = (int)(s % qtiles);
+    const int64_t r_begin = a.row_begin + rc * chunk_rows;
+    const int64_t r_end = r_begin + chunk_rows < a.row_end ? r_begin + chunk_rows : a.row_end;
 
     constexpr int QT = 16 * QS;
     const int q0 = qtile * QT;
@@ -339,12 +443,36 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     float* thr_s = reinterpret_cast<float*>(smem + QT * ldq);
     float* qn_s = thr_s + QT;
     float* qsc_s = qn_s + QT;
-    for (int idx = threadIdx.x; idx < QT * (qbytes / 16); idx += 64 * kWideWaves) {
-        int r = idx / (qbytes / 16);
-        int c = idx - r * (qbytes / 16);
-        i32x4 v = (i32x4){0, 0, 0, 0};
-        if (q0 + r < a.B) v = reinterpret_cast<const i32x4*>(static_cast<const uint8_t*>(a.Q) + (int64_t)(q0 + r) * qbytes)[c];
-        *reinterpret_cast<i32x4*>(smem + r * ldq + c * 16) = v;
+    uint32_t* cnt_s = reinterpret_cast<uint32_t*>(qsc_s + QT);  // [kWideWaves]
+    WideQueue wq;
+    wq.row = cnt_s + kWideWaves + (int64_t)wave * 2 * qcap;
+    wq.key = reinterpret_cast<float*>(wq.row + qcap);
+    wq.count = cnt_s + wave;
+    wq.cap = qcap;
+
+    // stage the query tile: eight 16-byte loads per lane in flight
+    {
+        const int per_row = qbytes / 16;
+        const int total = QT * per_row;
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < total; i0 += 64 * kWideWaves * U) {
+            i32x4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = i0 + u * 64 * kWideWaves + (int)threadIdx.x;
+                const int r = idx / per_row;
+                v[u] = (i32x4){0, 0, 0, 0};
+                if (idx < total && q0 + r < a.B)
+                    v[u] = reinterpret_cast<const i32x4*>(static_cast<const uint8_t*>(a.Q) +
+                                                          (int64_t)(q0 + r) * qbytes)[idx - r * per_row];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = i0 + u * 64 * kWideWaves + (int)threadIdx.x;
+                const int r = idx / per_row;
+                if (idx < total) *reinterpret_cast<i32x4*>(smem + r * ldq + (idx - r * per_row) * 16) = v[u];
+            }
+        }
     }
     for (int m = threadIdx.x; m < QT; m += 64 * kWideWaves) {
         int qi = q0 + m;
@@ -353,118 +481,156 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
         qn_s[m] = ok ? a.qn[qi] : 0.f;
         qsc_s[m] = (ok && a.qscale) ? a.qscale[qi] : 1.f;
     }
+    if (threadIdx.x < kWideWaves) cnt_s[threadIdx.x] = 0u;
     __syncthreads();
 
     const uint8_t* X = static_cast<const uint8_t*>(a.X);
     const int nchunk = a.kdim / P::kChunk;
     const int lane_dim = P::kDimsPerLoad * g;
     constexpr int TR = 16 * RS;  // rows per wave step
+    // two waves a SIMD (8-wave groups) leave half the registers: at most 4 chunks
+    constexpr int D = (kWideWaves == 8 && P::kRing > 4) ? 4 : P::kRing;
+    static_assert(D % 2 == 0, "query fragments alternate between two buffers per ring slot");
+    constexpr int CB = chunk_bytes<P>();
+    const int npad = (nchunk + D - 1) / D * D;  // chunks per tile, rounded up to whole ring turns
     const int64_t ntiles = (r_end - r_begin + TR - 1) / TR;
-    unsigned long long best[QS][4];
+    if (wave >= ntiles) return;
+
+    // Row fragments stream through a ring of D chunks: the chunk sequence of
+    // the wave's tiles is one stream (tile t chunks 0..npad-1, then tile
+    // t + waves), and the load of stream position j + D is issued right
+    // after the MFMAs of position j, so D chunks are in flight across tile
+    // boundaries too.  Positions past the last chunk of a tile reload its
+    // last chunk and skip their MFMAs.  With one wave a SIMD, query
+    // fragments alternate between two register sets: slot s multiplies
+    // qa[s & 1] and fills qa[(s+1) & 1] with the next chunk's (chunk 0 after
+    // the last); with two waves a SIMD the other wave hides the LDS latency
+    // and the fragments are read where they are used (measured: +16% fp32,
+    // whose 8-wave groups are short of registers).
+    constexpr bool kQPre = kWideWaves == 4;
+    typedef typename RawOf<P>::T Raw;
+    Raw ring[D][RS];
+    const uint8_t* cur[RS];
+    const uint8_t* nxt[RS];
+    auto tile_ptrs = [&](int64_t t, const uint8_t* (&p)[RS]) {
 #pragma unroll
-    for (int q = 0; q < QS; ++q)
+        for (int r = 0; r < RS; ++r) {
+            int64_t row = r_begin + t * TR + 16 * r + n;
+            if (row >= r_end) row = r_end - 1;
+            p[r] = X + row * a.xstride + g * lane_bytes<P>();
+        }
+    };
+    auto qfrag_off = [&](int c) {
+        return P::kElemBytes == 0 ? (c * P::kChunk + lane_dim) : (c * P::kChunk + lane_dim) * P::kElemBytes;
+    };
+    tile_ptrs(wave, cur);
+    // issued in slot order (the scheduler may not reorder them: the loop's
+    // waits assume slot s was requested before slot s + 1)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) best[q][i] = ~0ull;
+    for (int s2 = 0; s2 < D; ++s2) {
+#pragma unroll
+        for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>(cur[r] + (int64_t)(s2 < nchunk ? s2 : nchunk - 1) * CB);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    i32x4 qa[2][QS];
+    if constexpr (kQPre) {
+#pragma unroll
+        for (int q = 0; q < QS; ++q) qa[0][q] = *reinterpret_cast<const i32x4*>(smem + (16 * q + n) * ldq + qfrag_off(0));
+    }
 
     for (int64_t t = wave; t < ntiles; t += kWideWaves) {
         const int64_t r0 = r_begin + t * TR;
-        int64_t lrow[RS];
+        tile_ptrs(t, cur);
+        tile_ptrs(t + kWideWaves < ntiles ? t + kWideWaves : t, nxt);
+        // row norms, requested before this tile's chunk prefetches so that
+        // the epilogue's wait for them leaves those in flight
         bool rok[RS];
+        float xn[RS];
 #pragma unroll
         for (int r = 0; r < RS; ++r) {
-            int64_t row = r0 + 16 * r + n;
+            const int64_t row = r0 + 16 * r + n;
             rok[r] = row < r_end;
-            lrow[r] = rok[r] ? row : r_end - 1;
+            xn[r] = a.xn[rok[r] ? row : r_end - 1];
         }
         typename P::Acc acc[QS][RS];
 #pragma unroll
         for (int q = 0; q < QS; ++q)
 #pragma unroll
             for (int r = 0; r < RS; ++r) acc[q][r] = typename P::Acc{};
-        i32x4 bc[RS], bn[RS];
+        // do-while: the compiler then sees this tile's ring loads issued
+        // after the norm loads above on every path, and the epilogue's wait
+        // for the norms leaves the ring loads in flight
+        int c0 = 0;
+        do {
 #pragma unroll
-        for (int r = 0; r < RS; ++r) bc[r] = load_row_frag<P>(X, a.xstride, lrow[r], lane_dim);
-        for (int c = 0; c < nchunk; ++c) {
-            if (c + 1 < nchunk) {
+            for (int s2 = 0; s2 < D; ++s2) {
+                const int c = c0 + s2;
+                if constexpr (kQPre) {
+                    const int cq = c + 1 < nchunk ? c + 1 : 0;
 #pragma unroll
-                for (int r = 0; r < RS; ++r) bn[r] = load_row_frag<P>(X, a.xstride, lrow[r], (c + 1) * P::kChunk + lane_dim);
-            }
-            const int eoff = P::kElemBytes == 0 ? (c * P::kChunk + lane_dim) : (c * P::kChunk + lane_dim) * P::kElemBytes;
-#pragma unroll
-            for (int q = 0; q < QS; ++q) {
-                i32x4 qa = *reinterpret_cast<const i32x4*>(smem + (16 * q + n) * ldq + eoff);
-#pragma unroll
-                for (int r = 0; r < RS; ++r) P::mma(acc[q][r], qa, bc[r]);
-            }
-#pragma unroll
-            for (int r = 0; r < RS; ++r) bc[r] = bn[r];
-        }
-        // epilogue: cheap test first, append only when something passes
-        float xn[RS];
-#pragma unroll
-        for (int r = 0; r < RS; ++r) xn[r] = a.xn[lrow[r]];
-        if (a.argmin) {
-#pragma unroll
-            for (int q = 0; q < QS; ++q)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int m = 16 * q + 4 * g + i;
-                    const float qv = qn_s[m], sc = 2.0f * qsc_s[m];
-#pragma unroll
-                    for (int r = 0; r < RS; ++r) {
-                        float dist = xn[r] + qv - sc * (float)acc[q][r][i];
-                        unsigned long long kv = ((unsigned long long)f2key(dist) << 32) | (uint32_t)(r0 + 16 * r + n);
-                        if (rok[r] && kv < best[q][i]) best[q][i] = kv;
-                    }
+                    for (int q = 0; q < QS; ++q)
+                        qa[(s2 + 1) & 1][q] =
+                            *reinterpret_cast<const i32x4*>(smem + (16 * q + n) * ldq + qfrag_off(cq));
                 }
-            continue;
-        }
-        bool any = false;
-        v4f dist4[QS][RS];
+                if (c < nchunk) {
+                    if constexpr (!kQPre) {
 #pragma unroll
-        for (int q = 0; q < QS; ++q)
+                        for (int q = 0; q < QS; ++q)
+                            qa[s2 & 1][q] = *reinterpret_cast<const i32x4*>(smem + (16 * q + n) * ldq + qfrag_off(c));
+                    }
+                    i32x4 bb[RS];
+#pragma unroll
+                    for (int r = 0; r < RS; ++r) bb[r] = to_frag<P>(ring[s2][r]);
+#pragma unroll
+                    for (int q = 0; q < QS; ++q)
+#pragma unroll
+                        for (int r = 0; r < RS; ++r) P::mma(acc[q][r], qa[s2 & 1][q], bb[r]);
+                }
+                const int cn = c + D;
+                const bool same = cn < npad;
+                int cc = same ? cn : cn - npad;
+                cc = cc < nchunk ? cc : nchunk - 1;
+#pragma unroll
+                for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>((same ? cur[r] : nxt[r]) + (int64_t)cc * CB);
+            }
+            c0 += D;
+        } while (c0 < npad);
+
+        // epilogue: per 16-query group a cheap test, appends only where
+        // something passes (lanes diverge there)
+#pragma unroll
+        for (int q = 0; q < QS; ++q) {
+            float dist[4][RS];
+            bool pass[4][RS];
+            bool any = false;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int m = 16 * q + 4 * g + i;
                 const float th = thr_s[m], qv = qn_s[m], sc = 2.0f * qsc_s[m];
 #pragma unroll
                 for (int r = 0; r < RS; ++r) {
-                    float dist = xn[r] + qv - sc * (float)acc[q][r][i];
-                    dist4[q][r][i] = dist;
-                    any |= rok[r] && dist <= th;
+                    dist[i][r] = xn[r] + qv - sc * (float)acc[q][r][i];
+                    pass[i][r] = rok[r] && dist[i][r] <= th;
+                    any |= pass[i][r];
                 }
             }
-        if (__ballot(any) == 0) continue;
+            if (any) {
 #pragma unroll
-        for (int q = 0; q < QS; ++q)
+                for (int i = 0; i < 4; ++i) {
+                    const int m = 16 * q + 4 * g + i;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int m = 16 * q + 4 * g + i;
-                const float th = thr_s[m];
-                const int qi = q0 + m;
-#pragma unroll
-                for (int r = 0; r < RS; ++r) {
-                    const float dist = dist4[q][r][i];
-                    const bool pass = rok[r] && dist <= th;
-                    uint64_t mb = __ballot(pass);
-                    if (mb == 0) continue;
-                    uint32_t gm = (uint32_t)(mb >> (16 * g)) & 0xFFFFu;
-                    uint32_t base = 0;
-                    if (n == 0 && gm) base = atomicAdd(&a.cand.cnt[qi], (uint32_t)__builtin_popcount(gm));
-                    base = __shfl(base, lane & ~15);
-                    if (pass) {
-                        uint32_t off = base + __builtin_popcount(gm & ((1u << n) - 1u));
-                        if (off < (uint32_t)a.cand.cap) {
-                            int64_t sl = (int64_t)qi * a.cand.cap + off;
-                            const uint32_t row = (uint32_t)(r0 + 16 * r + n);
-                            a.cand.key[sl] = dist;
-                            a.cand.rid[sl] = a.rid_map ? a.rid_map[row] : row;
+                    for (int r = 0; r < RS; ++r)
+                        if (pass[i][r]) {
+                            const uint32_t roff = (uint32_t)(r0 - r_begin + 16 * r + n);
+                            wide_append(a, wq, q0 + m, m, roff, (uint32_t)(r_begin + roff), dist[i][r]);
                         }
-                    }
                 }
             }
+        }
+        // keep room for a tile's typical appends
+        if (__builtin_amdgcn_readfirstlane(*wq.count) + 512u > (uint32_t)qcap) wide_flush(a, wq, q0, r_begin, lane);
     }
-    if (a.argmin) argmin_flush<QS>(best, a, q0, g, n);
+    wide_flush(a, wq, q0, r_begin, lane);
 }
 
 template <class P, int QS, int RS, int NW>
@@ -479,7 +645,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
             ncu = 256;
     }
     constexpr int QT = 16 * QS;
-    const size_t lds = (size_t)QT * qrow_bytes<P>(a.kdim) + 3 * QT * sizeof(float);
+    const size_t lds = (size_t)QT * qrow_bytes<P>(a.kdim) + 3 * QT * sizeof(float) + NW * sizeof(uint32_t);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide_kernel<P, QS, RS, NW>),
@@ -487,16 +653,26 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
         attr = true;
     }
     const int qtiles = (a.B + QT - 1) / QT;
-    int S = ncu / 8;                       // workgroup slots per XCD (one workgroup per CU)
-    int rpx = S / qtiles;                  // row ranges per XCD
-    if (rpx < 1) rpx = 1;
-    const int slots = qtiles * rpx;
     const int64_t rows = a.row_end - a.row_begin;
-    constexpr int TR = 16 * RS;
-    int64_t range_len = (rows + 8 * rpx - 1) / (8 * rpx);
-    range_len = (range_len + TR - 1) / TR * TR;
-    hipLaunchKernelGGL((scan_wide_kernel<P, QS, RS, NW>), dim3((unsigned)(8 * slots)), dim3(64 * NW), lds, st, a,
-                       qtiles, rpx, range_len);
+    // row chunk: about 8 workgroups per CU over the launch, 2K..32K rows,
+    // whole wave steps
+    constexpr int64_t step = 16 * RS * NW;
+    int64_t chunk = rows * qtiles / ((int64_t)ncu * 8);
+    chunk = chunk < 2048 ? 2048 : (chunk > 32768 ? 32768 : chunk);
+    chunk = (chunk + step - 1) / step * step;
+    const int64_t nrc = (rows + chunk - 1) / chunk;
+    const int64_t per = (nrc + 7) / 8;
+    // the rest of the LDS holds the waves' candidate queues (8 bytes an
+    // entry); below 1024 entries a wave, every pair goes straight to the lists
+    int qcap = (int)((160 * 1024 - lds) / (NW * 8)) / 64 * 64;
+    if (qcap > 4096) qcap = 4096;
+    if (const char* e = getenv("ANNMI_WIDE_QCAP")) {  // lower cap: tuning and tests
+        const int v = atoi(e);
+        if (v >= 0 && v < qcap) qcap = v / 64 * 64;
+    }
+    if (qcap < 1024) qcap = 0;
+    hipLaunchKernelGGL((scan_wide_kernel<P, QS, RS, NW>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
+                       lds + (size_t)NW * qcap * 8, st, a, qtiles, chunk, nrc, qcap);
     return hipGetLastError();
 }
 
@@ -537,7 +713,10 @@ hipError_t launch_scan(const ScanArgs& a, hipStream_t st, const char** variant) 
 
 hipError_t launch_scan_impl(const ScanArgs& a, hipStream_t st) {
     if (a.row_end <= a.row_begin || a.B <= 0) return hipSuccess;
-    if (a.B >= g_wide_min_batch && (a.row_end - a.row_begin) >= 65536) {
+    // argmin scans (k-means assignment: few rows, many queries) take the
+    // streaming kernel, whose per-lane minima the wide kernel has no
+    // registers left for
+    if (a.B >= g_wide_min_batch && (a.row_end - a.row_begin) >= 65536 && !a.argmin) {
         switch (a.kind) {
             case SCAN_F32: return launch_wide<PolF32>(a, st);
             case SCAN_BF16: return launch_wide<PolBF16>(a, st);

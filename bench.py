@@ -172,6 +172,37 @@ def bench_ivf(args):
     print(json.dumps(res), flush=True)
 
 
+def copy_calibration(gib: float):
+    """One device-to-device copy of a known size: what FETCH_SIZE and
+    WRITE_SIZE read for it gives their scale on this GPU."""
+    import torch
+    nbytes = int(gib * (1 << 30))
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes}))
+
+
+def measured_traffic(workload: str, mode: str, kernel: str):
+    """HBM bytes per launch of the dominant kernel, from the counter runs
+    summarised in profiles/traffic_<workload>_<mode>.json (FETCH_SIZE and
+    WRITE_SIZE over the timed launches of this same command, each scaled by
+    the counter's reading for a copy of known size).  None when absent or
+    taken for another kernel."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"traffic_{workload}_{mode}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel:
+        return None
+    return {"bytes_per_launch": int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]),
+            "source": os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +216,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain fp32 path")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="mixed-10m-384", choices=["mixed-10m-384", "ivf-10m-768"])
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="only copy this many GiB device to device and exit (HBM counter calibration)")
     ap.add_argument("--requests", type=int, default=1500, help="served IVF: requests per rate")
     ap.add_argument("--rates", type=float, nargs="*", default=[2000, 10000, 50000])
     args = ap.parse_args()
@@ -194,6 +227,8 @@ def main():
               "(the index would shard by rows or queries; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_gib > 0:
+        return copy_calibration(args.copy_gib)
     if args.workload == "ivf-10m-768":
         return bench_ivf(args)
 
@@ -288,6 +323,11 @@ def main():
         roof = {"bound": "mfma", "achieved": round(tfl, 2), "peak": peak_t, "unit": "TFLOP/s",
                 "frac": round(f_mfma, 4)}
     roof["traffic"] = None
+    roof["algorithmic_bytes_per_launch"] = int(top["bytes"] / max(top["launches"], 1))
+    tr = measured_traffic(f"mixed-{n // 1_000_000}m-{d}" if n >= 1_000_000 else f"mixed-{n}-{d}", mode, top["kernel"])
+    if tr is not None:
+        roof["traffic"] = tr["bytes_per_launch"]
+        roof["traffic_source"] = tr["source"]
     roof["kernel"] = top["kernel"]
     roof["launches"] = int(top["launches"])
     roof["avg_launch_ms"] = round(top["ms"] / max(top["launches"], 1), 4)

@@ -124,3 +124,27 @@ def test_large_batch_kernel_matches_small_batch_and_oracle():
         i1, d1 = idx.search(Q[i:i + 1], 100)
         np.testing.assert_array_equal(i1[0], ids[i])
         np.testing.assert_array_equal(d1[0], dd[i])
+
+
+@pytest.mark.parametrize("qcap", ["1024", "0"])
+def test_large_batch_kernel_candidate_queue_paths(qcap, monkeypatch):
+    """The wide kernel stages passing pairs in a per-wave LDS queue.  With the
+    queue capped at 1024 entries it is flushed many times per launch; at 0
+    the kernel appends straight to the lists.  Both give the answer of the
+    default queue size, which is the exact one."""
+    base, Q = data.workload("exact-1m-384", n=150_000, n_queries=256)
+    print(f"[case] exact wide queue cap={qcap} n=150000 B=256", flush=True)
+    idx = Index(base, mode="exact")
+    ids0, dd0 = idx.search(Q, 100)
+    monkeypatch.setenv("ANNMI_WIDE_QCAP", qcap)
+    ids, dd = idx.search(Q, 100)
+    np.testing.assert_array_equal(ids, ids0)
+    np.testing.assert_array_equal(dd, dd0)
+    check_exact(base, Q[::51], 100, ids[::51], dd[::51])
+    bf = Index(base, mode="bf16", rerank=300)
+    monkeypatch.delenv("ANNMI_WIDE_QCAP")
+    ib0, db0 = bf.search(Q, 100)
+    monkeypatch.setenv("ANNMI_WIDE_QCAP", qcap)
+    ib, db = bf.search(Q, 100)
+    np.testing.assert_array_equal(ib, ib0)
+    np.testing.assert_array_equal(db, db0)
