@@ -122,6 +122,8 @@ struct Handle {
     bool own_x32 = false;
     float* xn32 = nullptr;
     float maxnorm = 0.f;
+    // min of xn and max of |xn| over the rows (the wide scan's prefilter)
+    float xn_lo = -__builtin_inff(), xn_amax = __builtin_inff();
     int kind = 0;
     int kdim = 0;
     void* Xs = nullptr;  // scan base
@@ -449,6 +451,7 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
         a.thr = h->thr;
         a.B = B;
         a.cand = cb;
+        a.xn_amax = h->xn_amax;
         int e0 = pr.ev();
         const char* vname = nullptr;
         HIP_OR(launch_scan(a, st, &vname), false);
@@ -461,6 +464,29 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
 }
 
 bool ivf_build(Handle* h);
+
+// Bounds of the scan's row norm term over all rows.
+bool xn_bounds(Handle* h) {
+    uint32_t* mm = nullptr;
+    HIP_OR(hipMalloc(&mm, 2 * sizeof(uint32_t)), false);
+    bool ok = launch_fill_u32(mm, 0xFFFFFFFFu, 1, h->st) == hipSuccess &&
+              launch_fill_u32(mm + 1, 0u, 1, h->st) == hipSuccess &&
+              launch_dim_minmax(h->xn, 1, h->n, 1, mm, mm + 1, h->st) == hipSuccess;
+    uint32_t hm[2] = {0, 0};
+    ok = ok && hipMemcpyAsync(hm, mm, sizeof(hm), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
+         hipStreamSynchronize(h->st) == hipSuccess;
+    dfree(mm);
+    if (!ok) {
+        set_err("row norm bounds failed: %s", hipGetErrorString(hipGetLastError()));
+        return false;
+    }
+    const float lo = key2f(hm[0]), hi = key2f(hm[1]);
+    if (lo == lo && hi == hi) {  // NaN norms leave the prefilter off
+        h->xn_lo = lo;
+        h->xn_amax = std::max(std::fabs(lo), std::fabs(hi));
+    }
+    return true;
+}
 
 bool setup_mode(Handle* h) {
     hipStream_t st = h->st;
@@ -659,6 +685,7 @@ Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params*
     dfree(mk);
     h->maxnorm = key2f(hk);
     if (!setup_mode(h)) return fail();
+    if (!xn_bounds(h)) return fail();
     return h;
 }
 
@@ -779,6 +806,8 @@ Handle* clone_impl(Handle* src) {
     h->X32 = src->X32;
     h->xn32 = src->xn32;
     h->maxnorm = src->maxnorm;
+    h->xn_lo = src->xn_lo;
+    h->xn_amax = src->xn_amax;
     h->kind = src->kind;
     h->kdim = src->kdim;
     h->Xs = src->Xs;

@@ -78,6 +78,9 @@ struct ScanArgs {
     unsigned long long* argmin = nullptr;
     // candidate ids are rid_map[row] when set (list-ordered rows -> original ids)
     const uint32_t* rid_map = nullptr;
+    // upper bound of |xn| over all rows (the wide scan's prefilter margin);
+    // the default disables the prefilter (every pair takes the exact test)
+    float xn_amax = __builtin_inff();
 };
 // Launches the scan; *variant (if not null) receives a static name of the
 // kernel instantiation launched (for per-kernel timing).

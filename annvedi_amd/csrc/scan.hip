@@ -28,6 +28,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 
 namespace {
 
@@ -48,6 +49,11 @@ struct PolF32 {
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[2], bf[2], acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[3], bf[3], acc, 0, 0, 0);
     }
+    // acc = a.b (zero C operand)
+    __device__ static inline void mma0(v4f& acc, const i32x4& a, const i32x4& b) {
+        acc = (v4f){0.f, 0.f, 0.f, 0.f};
+        mma(acc, a, b);
+    }
 };
 
 struct PolBF16 {
@@ -59,6 +65,10 @@ struct PolBF16 {
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
                                                       __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+    }
+    __device__ static inline void mma0(v4f& acc, const i32x4& a, const i32x4& b) {
+        acc = (v4f){0.f, 0.f, 0.f, 0.f};
+        mma(acc, a, b);
     }
 };
 
@@ -76,6 +86,10 @@ struct PolFP8 {
         acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a0, b0, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a1, b1, acc, 0, 0, 0);
     }
+    __device__ static inline void mma0(v4f& acc, const i32x4& a, const i32x4& b) {
+        acc = (v4f){0.f, 0.f, 0.f, 0.f};
+        mma(acc, a, b);
+    }
 };
 
 // int8 MFMA with an int32 accumulator.
@@ -91,6 +105,10 @@ struct PolSQ8 {
         i32x4 bs = b ^ (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bs, acc, 0, 0, 0);
     }
+    __device__ static inline void mma0(i32x4& acc, const i32x4& a, const i32x4& b) {
+        acc = (i32x4){0, 0, 0, 0};
+        mma(acc, a, b);
+    }
 };
 
 // SQ4: 8 packed bytes per lane per chunk of 64 dims.  Unpacked as
@@ -100,10 +118,14 @@ struct PolSQ4 {
     static constexpr int kDimsPerLoad = 16;
     static constexpr int kChunk = 64;
     static constexpr int kElemBytes = 0;  // packed: 2 dims per byte
-    static constexpr int kRing = 8;       // chunks of row fragments in flight per wave (wide scan)
+    static constexpr int kRing = 4;       // chunks of row fragments in flight per wave (wide scan)
     typedef i32x4 Acc;
     __device__ static inline void mma(i32x4& acc, const i32x4& a, const i32x4& b) {
         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
+    }
+    __device__ static inline void mma0(i32x4& acc, const i32x4& a, const i32x4& b) {
+        acc = (i32x4){0, 0, 0, 0};
+        mma(acc, a, b);
     }
 };
 
@@ -443,7 +465,8 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     float* thr_s = reinterpret_cast<float*>(smem + QT * ldq);
     float* qn_s = thr_s + QT;
     float* qsc_s = qn_s + QT;
-    uint32_t* cnt_s = reinterpret_cast<uint32_t*>(qsc_s + QT);  // [kWideWaves]
+    float* bnd_s = qsc_s + QT;
+    uint32_t* cnt_s = reinterpret_cast<uint32_t*>(bnd_s + QT);  // [kWideWaves]
     WideQueue wq;
     wq.row = cnt_s + kWideWaves + (int64_t)wave * 2 * qcap;
     wq.key = reinterpret_cast<float*>(wq.row + qcap);
@@ -477,9 +500,19 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     for (int m = threadIdx.x; m < QT; m += 64 * kWideWaves) {
         int qi = q0 + m;
         bool ok = qi < a.B;
-        thr_s[m] = ok ? a.thr[qi] : -__builtin_inff();
-        qn_s[m] = ok ? a.qn[qi] : 0.f;
-        qsc_s[m] = (ok && a.qscale) ? a.qscale[qi] : 1.f;
+        const float th = ok ? a.thr[qi] : -__builtin_inff();
+        const float qv = ok ? a.qn[qi] : 0.f;
+        const float qs = (ok && a.qscale) ? a.qscale[qi] : 1.f;
+        thr_s[m] = th;
+        qn_s[m] = qv;
+        qsc_s[m] = qs;
+        // Prefilter bound: a pair passes dist = (xn + qv) - 2*qs*dot <= th
+        // only if 2*qs*dot - xn >= qv - th - (rounding), so that value
+        // reaching bnd_s[m] is necessary.  The 2^-14 relative margin is
+        // ~1000x the fp32 rounding of the test; the exact test decides.
+        // th = -inf (padding) gives NaN: never passes; th = +inf: -inf.
+        const float M = a.xn_amax + fabsf(qv) + fabsf(th);
+        bnd_s[m] = (qv - th) - M * 0x1p-14f;
     }
     if (threadIdx.x < kWideWaves) cnt_s[threadIdx.x] = 0u;
     __syncthreads();
@@ -488,8 +521,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     const int nchunk = a.kdim / P::kChunk;
     const int lane_dim = P::kDimsPerLoad * g;
     constexpr int TR = 16 * RS;  // rows per wave step
-    // two waves a SIMD (8-wave groups) leave half the registers: at most 4 chunks
-    constexpr int D = (kWideWaves == 8 && P::kRing > 4) ? 4 : P::kRing;
+    constexpr int D = P::kRing;
     static_assert(D % 2 == 0, "query fragments alternate between two buffers per ring slot");
     constexpr int CB = chunk_bytes<P>();
     const int npad = (nchunk + D - 1) / D * D;  // chunks per tile, rounded up to whole ring turns
@@ -507,29 +539,32 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     // the last); with two waves a SIMD the other wave hides the LDS latency
     // and the fragments are read where they are used (measured: +16% fp32,
     // whose 8-wave groups are short of registers).
+    // Loads address rows by 32-bit byte offsets from the chunk's first row
+    // (a uniform base, so the offset is the only per-lane operand).
     constexpr bool kQPre = kWideWaves == 4;
     typedef typename RawOf<P>::T Raw;
     Raw ring[D][RS];
-    const uint8_t* cur[RS];
-    const uint8_t* nxt[RS];
-    auto tile_ptrs = [&](int64_t t, const uint8_t* (&p)[RS]) {
+    uint32_t cur[RS], nxt[RS];
+    const uint8_t* xbase = X + r_begin * a.xstride;
+    auto tile_offs = [&](int64_t t, uint32_t (&o)[RS]) {
 #pragma unroll
         for (int r = 0; r < RS; ++r) {
-            int64_t row = r_begin + t * TR + 16 * r + n;
-            if (row >= r_end) row = r_end - 1;
-            p[r] = X + row * a.xstride + g * lane_bytes<P>();
+            int64_t row = t * TR + 16 * r + n;
+            if (row >= r_end - r_begin) row = r_end - r_begin - 1;
+            o[r] = (uint32_t)(row * a.xstride) + (uint32_t)(g * lane_bytes<P>());
         }
     };
     auto qfrag_off = [&](int c) {
         return P::kElemBytes == 0 ? (c * P::kChunk + lane_dim) : (c * P::kChunk + lane_dim) * P::kElemBytes;
     };
-    tile_ptrs(wave, cur);
+    tile_offs(wave, cur);
     // issued in slot order (the scheduler may not reorder them: the loop's
     // waits assume slot s was requested before slot s + 1)
 #pragma unroll
     for (int s2 = 0; s2 < D; ++s2) {
+        const uint8_t* xb = xbase + (int64_t)(s2 < nchunk ? s2 : nchunk - 1) * CB;
 #pragma unroll
-        for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>(cur[r] + (int64_t)(s2 < nchunk ? s2 : nchunk - 1) * CB);
+        for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>(xb + cur[r]);
         __builtin_amdgcn_sched_barrier(0);
     }
     i32x4 qa[2][QS];
@@ -540,8 +575,8 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
 
     for (int64_t t = wave; t < ntiles; t += kWideWaves) {
         const int64_t r0 = r_begin + t * TR;
-        tile_ptrs(t, cur);
-        tile_ptrs(t + kWideWaves < ntiles ? t + kWideWaves : t, nxt);
+        tile_offs(t, cur);
+        tile_offs(t + kWideWaves < ntiles ? t + kWideWaves : t, nxt);
         // row norms, requested before this tile's chunk prefetches so that
         // the epilogue's wait for them leaves those in flight
         bool rok[RS];
@@ -553,15 +588,12 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
             xn[r] = a.xn[rok[r] ? row : r_end - 1];
         }
         typename P::Acc acc[QS][RS];
-#pragma unroll
-        for (int q = 0; q < QS; ++q)
-#pragma unroll
-            for (int r = 0; r < RS; ++r) acc[q][r] = typename P::Acc{};
-        // do-while: the compiler then sees this tile's ring loads issued
-        // after the norm loads above on every path, and the epilogue's wait
-        // for the norms leaves the ring loads in flight
-        int c0 = 0;
-        do {
+        // One ring turn (D chunks).  The first turn of a tile starts the
+        // accumulators from a zero C operand instead of clearing them.
+        auto turn = [&](int c0, auto first) {
+            constexpr bool kFirst = decltype(first)::value;
+            // all slots of a turn prefetch from the same tile
+            const bool same = c0 + D < npad;
 #pragma unroll
             for (int s2 = 0; s2 < D; ++s2) {
                 const int c = c0 + s2;
@@ -584,46 +616,52 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
 #pragma unroll
                     for (int q = 0; q < QS; ++q)
 #pragma unroll
-                        for (int r = 0; r < RS; ++r) P::mma(acc[q][r], qa[s2 & 1][q], bb[r]);
+                        for (int r = 0; r < RS; ++r) {
+                            if (kFirst && s2 == 0)
+                                P::mma0(acc[q][r], qa[s2 & 1][q], bb[r]);
+                            else
+                                P::mma(acc[q][r], qa[s2 & 1][q], bb[r]);
+                        }
                 }
                 const int cn = c + D;
-                const bool same = cn < npad;
                 int cc = same ? cn : cn - npad;
                 cc = cc < nchunk ? cc : nchunk - 1;
+                const uint8_t* xb = xbase + (int64_t)cc * CB;
 #pragma unroll
-                for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>((same ? cur[r] : nxt[r]) + (int64_t)cc * CB);
+                for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>(xb + (same ? cur[r] : nxt[r]));
             }
-            c0 += D;
-        } while (c0 < npad);
+        };
+        turn(0, std::true_type{});
+        for (int c0 = D; c0 < npad; c0 += D) turn(c0, std::false_type{});
 
-        // epilogue: per 16-query group a cheap test, appends only where
-        // something passes (lanes diverge there)
+        // epilogue.  Prefilter per (query, lane): the largest 2*qs*dot - xn
+        // over the lane's RS rows against the query's bound; only lanes and
+        // query groups that may pass evaluate the exact test and append.
 #pragma unroll
         for (int q = 0; q < QS; ++q) {
-            float dist[4][RS];
-            bool pass[4][RS];
-            bool any = false;
+            bool maybe = false;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int m = 16 * q + 4 * g + i;
-                const float th = thr_s[m], qv = qn_s[m], sc = 2.0f * qsc_s[m];
+                const float sc = 2.0f * qsc_s[m];
+                float mx = fmaf(sc, (float)acc[q][0][i], -xn[0]);
 #pragma unroll
-                for (int r = 0; r < RS; ++r) {
-                    dist[i][r] = xn[r] + qv - sc * (float)acc[q][r][i];
-                    pass[i][r] = rok[r] && dist[i][r] <= th;
-                    any |= pass[i][r];
-                }
+                for (int r = 1; r < RS; ++r) mx = fmaxf(mx, fmaf(sc, (float)acc[q][r][i], -xn[r]));
+                maybe |= mx >= bnd_s[m];
             }
-            if (any) {
+            if (maybe) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int m = 16 * q + 4 * g + i;
+                    const float th = thr_s[m], qv = qn_s[m], sc = 2.0f * qsc_s[m];
 #pragma unroll
-                    for (int r = 0; r < RS; ++r)
-                        if (pass[i][r]) {
+                    for (int r = 0; r < RS; ++r) {
+                        const float dist = xn[r] + qv - sc * (float)acc[q][r][i];
+                        if (rok[r] && dist <= th) {
                             const uint32_t roff = (uint32_t)(r0 - r_begin + 16 * r + n);
-                            wide_append(a, wq, q0 + m, m, roff, (uint32_t)(r_begin + roff), dist[i][r]);
+                            wide_append(a, wq, q0 + m, m, roff, (uint32_t)(r_begin + roff), dist);
                         }
+                    }
                 }
             }
         }
@@ -645,7 +683,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
             ncu = 256;
     }
     constexpr int QT = 16 * QS;
-    const size_t lds = (size_t)QT * qrow_bytes<P>(a.kdim) + 3 * QT * sizeof(float) + NW * sizeof(uint32_t);
+    const size_t lds = (size_t)QT * qrow_bytes<P>(a.kdim) + 4 * QT * sizeof(float) + NW * sizeof(uint32_t);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide_kernel<P, QS, RS, NW>),
