@@ -439,7 +439,7 @@ __device__ inline void wide_flush(const ScanArgs& a, const WideQueue& wq, int q0
     __builtin_amdgcn_wave_barrier();
 }
 
-template <class P, int QS, int RS, int kWideWaves>
+template <class P, int QS, int RS, int kWideWaves, int QH, bool kQPre>
 __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, int qtiles, int64_t chunk_rows,
                                                                     int64_t nrc, int qcap) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -458,7 +458,12 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     const int64_t r_begin = a.row_begin + rc * chunk_rows;
     const int64_t r_end = r_begin + chunk_rows < a.row_end ? r_begin + chunk_rows : a.row_end;
 
-    constexpr int QT = 16 * QS;
+    // QH > 1: the waves split the query tile QH ways and share row tiles
+    constexpr int QT = 16 * QS * QH;
+    constexpr int NS = kWideWaves / QH;  // row streams
+    static_assert(QT <= 128, "queue entries hold the query slot in 7 bits");
+    const int qbase = 16 * QS * (wave % QH);
+    const int stream = wave / QH;
     const int q0 = qtile * QT;
     const int qbytes = P::kElemBytes == 0 ? a.kdim : a.kdim * P::kElemBytes;
     const int ldq = qrow_bytes<P>(a.kdim);
@@ -526,22 +531,20 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     constexpr int CB = chunk_bytes<P>();
     const int npad = (nchunk + D - 1) / D * D;  // chunks per tile, rounded up to whole ring turns
     const int64_t ntiles = (r_end - r_begin + TR - 1) / TR;
-    if (wave >= ntiles) return;
+    if (stream >= ntiles) return;
 
     // Row fragments stream through a ring of D chunks: the chunk sequence of
     // the wave's tiles is one stream (tile t chunks 0..npad-1, then tile
-    // t + waves), and the load of stream position j + D is issued right
-    // after the MFMAs of position j, so D chunks are in flight across tile
+    // t + NS), and the load of stream position j + D is issued right after
+    // the MFMAs of position j, so D chunks are in flight across tile
     // boundaries too.  Positions past the last chunk of a tile reload its
-    // last chunk and skip their MFMAs.  With one wave a SIMD, query
-    // fragments alternate between two register sets: slot s multiplies
-    // qa[s & 1] and fills qa[(s+1) & 1] with the next chunk's (chunk 0 after
-    // the last); with two waves a SIMD the other wave hides the LDS latency
-    // and the fragments are read where they are used (measured: +16% fp32,
-    // whose 8-wave groups are short of registers).
+    // last chunk and skip their MFMAs.  With kQPre, query fragments
+    // alternate between two register sets: slot s multiplies qa[s & 1] and
+    // fills qa[(s+1) & 1] with the next chunk's (chunk 0 after the last);
+    // otherwise they are read where they are used and a second wave on the
+    // SIMD hides the LDS latency.
     // Loads address rows by 32-bit byte offsets from the chunk's first row
     // (a uniform base, so the offset is the only per-lane operand).
-    constexpr bool kQPre = kWideWaves == 4;
     typedef typename RawOf<P>::T Raw;
     Raw ring[D][RS];
     uint32_t cur[RS], nxt[RS];
@@ -557,7 +560,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     auto qfrag_off = [&](int c) {
         return P::kElemBytes == 0 ? (c * P::kChunk + lane_dim) : (c * P::kChunk + lane_dim) * P::kElemBytes;
     };
-    tile_offs(wave, cur);
+    tile_offs(stream, cur);
     // issued in slot order (the scheduler may not reorder them: the loop's
     // waits assume slot s was requested before slot s + 1)
 #pragma unroll
@@ -570,13 +573,14 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     i32x4 qa[2][QS];
     if constexpr (kQPre) {
 #pragma unroll
-        for (int q = 0; q < QS; ++q) qa[0][q] = *reinterpret_cast<const i32x4*>(smem + (16 * q + n) * ldq + qfrag_off(0));
+        for (int q = 0; q < QS; ++q)
+            qa[0][q] = *reinterpret_cast<const i32x4*>(smem + (qbase + 16 * q + n) * ldq + qfrag_off(0));
     }
 
-    for (int64_t t = wave; t < ntiles; t += kWideWaves) {
+    for (int64_t t = stream; t < ntiles; t += NS) {
         const int64_t r0 = r_begin + t * TR;
         tile_offs(t, cur);
-        tile_offs(t + kWideWaves < ntiles ? t + kWideWaves : t, nxt);
+        tile_offs(t + NS < ntiles ? t + NS : t, nxt);
         // row norms, requested before this tile's chunk prefetches so that
         // the epilogue's wait for them leaves those in flight
         bool rok[RS];
@@ -602,13 +606,14 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
 #pragma unroll
                     for (int q = 0; q < QS; ++q)
                         qa[(s2 + 1) & 1][q] =
-                            *reinterpret_cast<const i32x4*>(smem + (16 * q + n) * ldq + qfrag_off(cq));
+                            *reinterpret_cast<const i32x4*>(smem + (qbase + 16 * q + n) * ldq + qfrag_off(cq));
                 }
                 if (c < nchunk) {
                     if constexpr (!kQPre) {
 #pragma unroll
                         for (int q = 0; q < QS; ++q)
-                            qa[s2 & 1][q] = *reinterpret_cast<const i32x4*>(smem + (16 * q + n) * ldq + qfrag_off(c));
+                            qa[s2 & 1][q] =
+                                *reinterpret_cast<const i32x4*>(smem + (qbase + 16 * q + n) * ldq + qfrag_off(c));
                     }
                     i32x4 bb[RS];
 #pragma unroll
@@ -642,7 +647,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
             bool maybe = false;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int m = 16 * q + 4 * g + i;
+                const int m = qbase + 16 * q + 4 * g + i;
                 const float sc = 2.0f * qsc_s[m];
                 float mx = fmaf(sc, (float)acc[q][0][i], -xn[0]);
 #pragma unroll
@@ -652,7 +657,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
             if (maybe) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int m = 16 * q + 4 * g + i;
+                    const int m = qbase + 16 * q + 4 * g + i;
                     const float th = thr_s[m], qv = qn_s[m], sc = 2.0f * qsc_s[m];
 #pragma unroll
                     for (int r = 0; r < RS; ++r) {
@@ -671,10 +676,12 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     wide_flush(a, wq, q0, r_begin, lane);
 }
 
-template <class P, int QS, int RS, int NW>
+template <class P, int QS, int RS, int NW, int QH, bool QPRE>
 hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
-    static char name[64] = "";
-    if (!name[0]) snprintf(name, sizeof(name), "scan_wide_kernel<%s,QS=%d,RS=%d,NW=%d>", pol_name<P>(), QS, RS, NW);
+    static char name[80] = "";
+    if (!name[0])
+        snprintf(name, sizeof(name), "scan_wide_kernel<%s,QS=%d,RS=%d,NW=%d,QH=%d%s>", pol_name<P>(), QS, RS, NW, QH,
+                 QPRE ? ",QP" : "");
     t_variant = name;
     static int ncu = 0;
     if (ncu == 0) {
@@ -682,11 +689,11 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             ncu = 256;
     }
-    constexpr int QT = 16 * QS;
+    constexpr int QT = 16 * QS * QH;
     const size_t lds = (size_t)QT * qrow_bytes<P>(a.kdim) + 4 * QT * sizeof(float) + NW * sizeof(uint32_t);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide_kernel<P, QS, RS, NW>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide_kernel<P, QS, RS, NW, QH, QPRE>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
@@ -694,7 +701,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     const int64_t rows = a.row_end - a.row_begin;
     // row chunk: about 8 workgroups per CU over the launch, 2K..32K rows,
     // whole wave steps
-    constexpr int64_t step = 16 * RS * NW;
+    constexpr int64_t step = 16 * RS * (NW / QH);
     int64_t chunk = rows * qtiles / ((int64_t)ncu * 8);
     chunk = chunk < 2048 ? 2048 : (chunk > 32768 ? 32768 : chunk);
     chunk = (chunk + step - 1) / step * step;
@@ -709,7 +716,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
         if (v >= 0 && v < qcap) qcap = v / 64 * 64;
     }
     if (qcap < 1024) qcap = 0;
-    hipLaunchKernelGGL((scan_wide_kernel<P, QS, RS, NW>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
+    hipLaunchKernelGGL((scan_wide_kernel<P, QS, RS, NW, QH, QPRE>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
                        lds + (size_t)NW * qcap * 8, st, a, qtiles, chunk, nrc, qcap);
     return hipGetLastError();
 }
@@ -718,10 +725,13 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
 template <class P>
 hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
     const size_t row = (size_t)qrow_bytes<P>(a.kdim) + 12;
-    // 128-query tiles need ~300 registers per lane: one wave per SIMD
-    if (128 * row <= 160 * 1024) return launch_wide_t<P, 8, 4, 4>(a, st);
-    if (64 * row <= 160 * 1024) return launch_wide_t<P, 4, 4, 8>(a, st);
-    if (32 * row <= 160 * 1024) return launch_wide_t<P, 2, 4, 8>(a, st);
+    if (128 * row <= 160 * 1024) {
+        if constexpr (P::kElemBytes == 2) return launch_wide_t<P, 8, 4, 4, 1, true>(a, st);   // CFG_A
+        else if constexpr (P::kElemBytes == 4) return launch_wide_t<P, 8, 4, 4, 1, true>(a, st);
+        else return launch_wide_t<P, 8, 2, 8, 1, false>(a, st);                              // CFG_B
+    }
+    if (64 * row <= 160 * 1024) return launch_wide_t<P, 4, 4, 8, 1, false>(a, st);
+    if (32 * row <= 160 * 1024) return launch_wide_t<P, 2, 4, 8, 1, false>(a, st);
     return hipErrorInvalidValue;
 }
 
