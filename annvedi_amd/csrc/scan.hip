@@ -702,8 +702,13 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     // row chunk: about 8 workgroups per CU over the launch, 2K..32K rows,
     // whole wave steps
     constexpr int64_t step = 16 * RS * (NW / QH);
+    static const int64_t chunk_max = [] {  // tuning: ANNMI_WIDE_CHUNK_MAX rows
+        const char* e = getenv("ANNMI_WIDE_CHUNK_MAX");
+        const long v = e ? atol(e) : 0;
+        return v >= 2048 ? (int64_t)v : (int64_t)32768;
+    }();
     int64_t chunk = rows * qtiles / ((int64_t)ncu * 8);
-    chunk = chunk < 2048 ? 2048 : (chunk > 32768 ? 32768 : chunk);
+    chunk = chunk < 2048 ? 2048 : (chunk > chunk_max ? chunk_max : chunk);
     chunk = (chunk + step - 1) / step * step;
     const int64_t nrc = (rows + chunk - 1) / chunk;
     const int64_t per = (nrc + 7) / 8;
@@ -748,7 +753,22 @@ hipError_t launch_p(const ScanArgs& a, hipStream_t st) {
 
 }  // namespace
 
-int g_wide_min_batch = 128;
+// Batches from this size up take the wide kernel: the crossover measured on
+// 10M x 384 (the streaming kernel re-reads rows per 64-query tile and
+// appends with returning atomics; the wide kernel pays a query-tile staging
+// per row chunk).  ANNMI_WIDE_MIN_BATCH overrides it for every type (tuning).
+int wide_min_batch(int kind) {
+    static const int forced = [] {
+        const char* e = getenv("ANNMI_WIDE_MIN_BATCH");
+        return e ? atoi(e) : 0;
+    }();
+    if (forced > 0) return forced;
+    switch (kind) {
+        case SCAN_F32: return 33;
+        case SCAN_FP8: return 4;
+        default: return 8;  // bf16, sq8, sq4
+    }
+}
 
 hipError_t launch_scan_impl(const ScanArgs& a, hipStream_t st);
 
@@ -764,7 +784,7 @@ hipError_t launch_scan_impl(const ScanArgs& a, hipStream_t st) {
     // argmin scans (k-means assignment: few rows, many queries) take the
     // streaming kernel, whose per-lane minima the wide kernel has no
     // registers left for
-    if (a.B >= g_wide_min_batch && (a.row_end - a.row_begin) >= 65536 && !a.argmin) {
+    if (a.B >= wide_min_batch(a.kind) && (a.row_end - a.row_begin) >= 65536 && !a.argmin) {
         switch (a.kind) {
             case SCAN_F32: return launch_wide<PolF32>(a, st);
             case SCAN_BF16: return launch_wide<PolBF16>(a, st);

@@ -186,19 +186,19 @@ def copy_calibration(gib: float):
     print(json.dumps({"copy_bytes": nbytes}))
 
 
-def measured_traffic(workload: str, mode: str, kernel: str):
+def measured_traffic(workload: str, mode: str, kernel: str, launches: int):
     """HBM bytes per launch of the dominant kernel, from the counter runs
     summarised in profiles/traffic_<workload>_<mode>.json (FETCH_SIZE and
     WRITE_SIZE over the timed launches of this same command, each scaled by
     the counter's reading for a copy of known size).  None when absent or
-    taken for another kernel."""
+    taken for another kernel or launch count."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"traffic_{workload}_{mode}.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kernel:
+    if t.get("kernel") != kernel or t.get("launches") != launches:
         return None
     return {"bytes_per_launch": int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]),
             "source": os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))}
@@ -325,7 +325,8 @@ def main():
                 "frac": round(f_mfma, 4)}
     roof["traffic"] = None
     roof["algorithmic_bytes_per_launch"] = int(top["bytes"] / max(top["launches"], 1))
-    tr = measured_traffic(f"mixed-{n // 1_000_000}m-{d}" if n >= 1_000_000 else f"mixed-{n}-{d}", mode, top["kernel"])
+    tr = measured_traffic(f"mixed-{n // 1_000_000}m-{d}" if n >= 1_000_000 else f"mixed-{n}-{d}", mode, top["kernel"],
+                          int(top["launches"]))
     if tr is not None:
         roof["traffic"] = tr["bytes_per_launch"]
         roof["traffic_source"] = tr["source"]

@@ -113,7 +113,7 @@ def test_exact_one_million_rows():
 
 
 def test_large_batch_kernel_matches_small_batch_and_oracle():
-    """B >= 128 over >= 65536-row ranges runs the query-stationary kernel;
+    """B >= 33 over >= 65536-row ranges runs the query-stationary kernel;
     B = 1 runs the streaming kernel.  Both must give the exact answer."""
     base, Q = data.workload("exact-1m-384", n=150_000, n_queries=300)
     idx = Index(base, mode="exact")

@@ -68,7 +68,7 @@ def test_gpu_filter_abi_drop_in():
 
 @pytest.mark.parametrize("mode", ["bf16", "fp8", "sq8", "sq4"])
 def test_large_batch_quantized_consistent(mode):
-    """Query-stationary kernel (B >= 128) vs streaming kernel (B = 1): with
+    """Query-stationary kernel (B = 256) vs streaming kernel (B = 1): with
     R = 600 both re-score supersets of the true top-100 on this data, so the
     answers agree with each other and with the oracle."""
     base, Q = data.workload("exact-1m-384", n=120_000, n_queries=256)
