@@ -78,9 +78,13 @@ struct ScanArgs {
     unsigned long long* argmin = nullptr;
     // candidate ids are rid_map[row] when set (list-ordered rows -> original ids)
     const uint32_t* rid_map = nullptr;
-    // upper bound of |xn| over all rows (the wide scan's prefilter margin);
-    // the default disables the prefilter (every pair takes the exact test)
+    // bounds of xn over all rows: min, max of |xn| (the wide scan's
+    // prefilter; the defaults make every pair take the exact test), and the
+    // prefilter mode: 1 per-row bound, 2 one bound per query from xn_lo
+    // (chosen when the row norms are nearly equal)
+    float xn_lo = -__builtin_inff();
     float xn_amax = __builtin_inff();
+    int prefilter = 1;
 };
 // Launches the scan; *variant (if not null) receives a static name of the
 // kernel instantiation launched (for per-kernel timing).

@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void enc_sq8(const float* src, int64_t sstride
         if (j < d) {
             int qv = sq_round((p[j] - offset[j]) / scale[j], variant);
             c = (uint8_t)(qv < 0 ? 0 : (qv > 255 ? 255 : qv));
-            float v = (float)c * scale[j];
+            float v = fmaf((float)c, scale[j], offset[j]);  // reconstructed value
             s = fmaf(v, v, s);
         }
         dst[row * kdim + j] = c;
@@ -132,13 +132,13 @@ __global__ __launch_bounds__(256) void enc_sq4(const float* src, int64_t sstride
         if (dlo < d) {
             int v = (int)roundf((p[dlo] - offset[dlo]) / scale[dlo]);
             nlo = v < 0 ? 0 : (v > 15 ? 15 : v);
-            float f = (float)nlo * scale[dlo];
+            float f = fmaf((float)nlo, scale[dlo], offset[dlo]);
             s = fmaf(f, f, s);
         }
         if (dhi < d) {
             int v = (int)roundf((p[dhi] - offset[dhi]) / scale[dhi]);
             nhi = v < 0 ? 0 : (v > 15 ? 15 : v);
-            float f = (float)nhi * scale[dhi];
+            float f = fmaf((float)nhi, scale[dhi], offset[dhi]);
             s = fmaf(f, f, s);
         }
         dst[row * stride + b] = (uint8_t)(nlo | (nhi << 4));
@@ -276,16 +276,21 @@ __global__ __launch_bounds__(256) void prep_queries(QPrepArgs a) {
             a.qscale[q] = 1.0f / (a.alpha_x * aq);
             a.margin[q] = 0.f;
         }
-    } else {  // SQ8 / SQ4: int8 weights w = scale * (q - offset), per-query scale beta
-        float m = 0.f, qq = 0.f;
+    } else {
+        // SQ8 / SQ4 against the reconstructed rows x^ = offset + scale * c:
+        // ||x^ - q||^2 = ||x^||^2 + (||q||^2 - 2 q.offset) - 2 sum_d (scale_d q_d) c_d,
+        // the last sum as int8 weights w = scale * q with a per-query scale
+        // beta (row norm ||x^||^2 from the encoder).
+        float m = 0.f, qq = 0.f, qo = 0.f;
         for (int j = threadIdx.x; j < a.d; j += 256) {
-            float qs = x[j] - a.offset[j];
-            float w = a.scale[j] * qs;
+            float w = a.scale[j] * x[j];
             m = fmaxf(m, fabsf(w));
-            qq = fmaf(qs, qs, qq);
+            qq = fmaf(x[j], x[j], qq);
+            qo = fmaf(x[j], a.offset[j], qo);
         }
         m = block_reduce_max(m, red);
         qq = block_reduce_sum(qq, red);
+        qo = block_reduce_sum(qo, red);
         float beta = m > 0.f ? m / 127.0f : 1.0f;
         int8_t* o = static_cast<int8_t*>(a.Qscan) + (int64_t)q * a.kdim;
         float wsum = 0.f;
@@ -293,7 +298,7 @@ __global__ __launch_bounds__(256) void prep_queries(QPrepArgs a) {
             int j = a.kind == SCAN_SQ4 ? sq4_dim_of_slot(p) : p;
             int wi = 0;
             if (j < a.d) {
-                float w = a.scale[j] * (x[j] - a.offset[j]);
+                float w = a.scale[j] * x[j];
                 wi = (int)rintf(w / beta);
                 wi = wi < -127 ? -127 : (wi > 127 ? 127 : wi);
             }
@@ -304,7 +309,8 @@ __global__ __launch_bounds__(256) void prep_queries(QPrepArgs a) {
         if (threadIdx.x == 0) {
             // SQ8 codes enter the MFMA as c - 128: add 128 * sum(w) back
             // through the norm term: -2 beta (acc + 128 sum w).
-            a.qn[q] = a.kind == SCAN_SQ8 ? qq - 256.0f * beta * wsum : qq;
+            const float base = qq - 2.0f * qo;
+            a.qn[q] = a.kind == SCAN_SQ8 ? base - 256.0f * beta * wsum : base;
             a.qscale[q] = beta;
             a.margin[q] = 0.f;
         }

@@ -41,6 +41,7 @@ struct PolF32 {
     static constexpr int kElemBytes = 4;
     static constexpr int kRing = 2;       // chunks of row fragments in flight per wave (wide scan)
     typedef v4f Acc;
+    typedef float El;  // accumulator element
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         f32x4 af = __builtin_bit_cast(f32x4, a);
         f32x4 bf = __builtin_bit_cast(f32x4, b);
@@ -62,6 +63,7 @@ struct PolBF16 {
     static constexpr int kElemBytes = 2;
     static constexpr int kRing = 4;       // chunks of row fragments in flight per wave (wide scan)
     typedef v4f Acc;
+    typedef float El;  // accumulator element
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
                                                       __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
@@ -78,6 +80,7 @@ struct PolFP8 {
     static constexpr int kElemBytes = 1;
     static constexpr int kRing = 4;       // chunks of row fragments in flight per wave (wide scan)
     typedef v4f Acc;
+    typedef float El;  // accumulator element
     __device__ static inline void mma(v4f& acc, const i32x4& a, const i32x4& b) {
         long a0 = ((long)(uint32_t)a[1] << 32) | (uint32_t)a[0];
         long a1 = ((long)(uint32_t)a[3] << 32) | (uint32_t)a[2];
@@ -99,6 +102,7 @@ struct PolSQ8 {
     static constexpr int kElemBytes = 1;
     static constexpr int kRing = 4;       // chunks of row fragments in flight per wave (wide scan)
     typedef i32x4 Acc;
+    typedef int El;  // accumulator element
     __device__ static inline void mma(i32x4& acc, const i32x4& a, const i32x4& b) {
         // codes are uint8 in [0,255]; the MFMA reads signed int8, so flip the
         // top bit: c - 128.  The query side adds 128*sum(w) back.
@@ -120,6 +124,7 @@ struct PolSQ4 {
     static constexpr int kElemBytes = 0;  // packed: 2 dims per byte
     static constexpr int kRing = 4;       // chunks of row fragments in flight per wave (wide scan)
     typedef i32x4 Acc;
+    typedef int El;  // accumulator element
     __device__ static inline void mma(i32x4& acc, const i32x4& a, const i32x4& b) {
         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
     }
@@ -471,7 +476,9 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     float* qn_s = thr_s + QT;
     float* qsc_s = qn_s + QT;
     float* bnd_s = qsc_s + QT;
-    uint32_t* cnt_s = reinterpret_cast<uint32_t*>(bnd_s + QT);  // [kWideWaves]
+    typedef typename P::El El;
+    El* bndg_s = reinterpret_cast<El*>(bnd_s + QT);
+    uint32_t* cnt_s = reinterpret_cast<uint32_t*>(bndg_s + QT);  // [kWideWaves]
     WideQueue wq;
     wq.row = cnt_s + kWideWaves + (int64_t)wave * 2 * qcap;
     wq.key = reinterpret_cast<float*>(wq.row + qcap);
@@ -518,6 +525,18 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
         // th = -inf (padding) gives NaN: never passes; th = +inf: -inf.
         const float M = a.xn_amax + fabsf(qv) + fabsf(th);
         bnd_s[m] = (qv - th) - M * 0x1p-14f;
+        // global form (prefilter 2): xn >= xn_lo for every row, so
+        // dot >= ((xn_lo + qv) - th - margin) / (2 qs) is necessary
+        const float sc = 2.0f * qs;
+        const float num = ((a.xn_lo + qv) - th) - M * 0x1p-14f;
+        const float bg = sc > 0.f ? num / sc : (num <= 0.f ? -__builtin_inff() : __builtin_inff());
+        if constexpr (std::is_same<El, float>::value) {
+            bndg_s[m] = bg;
+        } else {  // integer dot: the smallest integer >= bg; NaN (padding) never passes
+            bndg_s[m] = bg != bg ? 0x7fffffff
+                                 : (bg <= -2147483648.0f ? (int)0x80000000
+                                                         : (bg >= 2147483647.0f ? 0x7fffffff : (int)ceilf(bg)));
+        }
     }
     if (threadIdx.x < kWideWaves) cnt_s[threadIdx.x] = 0u;
     __syncthreads();
@@ -640,19 +659,31 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
         for (int c0 = D; c0 < npad; c0 += D) turn(c0, std::false_type{});
 
         // epilogue.  Prefilter per (query, lane): the largest 2*qs*dot - xn
-        // over the lane's RS rows against the query's bound; only lanes and
-        // query groups that may pass evaluate the exact test and append.
+        // over the lane's RS rows against the query's bound (or, when all
+        // row norms are nearly equal, the largest dot against one bound per
+        // query); only lanes and query groups that may pass evaluate the
+        // exact test and append.
 #pragma unroll
         for (int q = 0; q < QS; ++q) {
             bool maybe = false;
+            if (a.prefilter == 2) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int m = qbase + 16 * q + 4 * g + i;
-                const float sc = 2.0f * qsc_s[m];
-                float mx = fmaf(sc, (float)acc[q][0][i], -xn[0]);
+                for (int i = 0; i < 4; ++i) {
+                    El mx = acc[q][0][i];
 #pragma unroll
-                for (int r = 1; r < RS; ++r) mx = fmaxf(mx, fmaf(sc, (float)acc[q][r][i], -xn[r]));
-                maybe |= mx >= bnd_s[m];
+                    for (int r = 1; r < RS; ++r) mx = acc[q][r][i] > mx ? acc[q][r][i] : mx;
+                    maybe |= mx >= bndg_s[qbase + 16 * q + 4 * g + i];
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int m = qbase + 16 * q + 4 * g + i;
+                    const float sc = 2.0f * qsc_s[m];
+                    float mx = fmaf(sc, (float)acc[q][0][i], -xn[0]);
+#pragma unroll
+                    for (int r = 1; r < RS; ++r) mx = fmaxf(mx, fmaf(sc, (float)acc[q][r][i], -xn[r]));
+                    maybe |= mx >= bnd_s[m];
+                }
             }
             if (maybe) {
 #pragma unroll
@@ -690,7 +721,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
             ncu = 256;
     }
     constexpr int QT = 16 * QS * QH;
-    const size_t lds = (size_t)QT * qrow_bytes<P>(a.kdim) + 4 * QT * sizeof(float) + NW * sizeof(uint32_t);
+    const size_t lds = (size_t)QT * qrow_bytes<P>(a.kdim) + 5 * QT * sizeof(float) + NW * sizeof(uint32_t);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide_kernel<P, QS, RS, NW, QH, QPRE>),

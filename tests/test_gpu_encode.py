@@ -70,7 +70,7 @@ def test_norm_terms():
     idx = Index(base, mode="sq8")
     s, o = O.sq8_train(base)
     codes = O.sq8_encode(base, s, o)
-    want = ((codes.astype(np.float64) * s) ** 2).sum(1)
+    want = ((codes.astype(np.float64) * s + o) ** 2).sum(1)  # ||reconstructed row||^2
     np.testing.assert_allclose(idx.export("norms"), want, rtol=1e-5)
     ex = Index(base, mode="exact")
     np.testing.assert_allclose(ex.export("norms"), (base.astype(np.float64) ** 2).sum(1), rtol=1e-5)
