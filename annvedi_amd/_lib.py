@@ -28,7 +28,8 @@ EXPORTS = [
     "annmi_default_params", "annmi_create", "annmi_query", "annmi_batch_query", "annmi_batch_query_device",
     "annmi_batch_query_ex", "annmi_query_linear_index",
     "annmi_ndist", "annmi_last_batch_stats", "annmi_free", "annmi_clone", "annmi_last_error", "annmi_index_bytes",
-    "annmi_profile", "annmi_profile_kernels", "annmi_export",
+    "annmi_profile", "annmi_profile_kernels", "annmi_export", "annmi_graph_count",
+    "annmi_autotune", "annmi_set_scan_shape",
     "gf_create", "gf_query", "gf_ndist", "gf_free", "gf_last_error",
 ]
 
@@ -101,6 +102,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.annmi_index_bytes.argtypes = [vp]
     L.annmi_profile_kernels.restype = i32
     L.annmi_profile_kernels.argtypes = [vp, ctypes.c_char_p, i32]
+    L.annmi_graph_count.restype = i32
+    L.annmi_graph_count.argtypes = [vp]
+    L.annmi_autotune.restype = i32
+    L.annmi_autotune.argtypes = [vp, i32, i32, ctypes.c_char_p, i32]
+    L.annmi_set_scan_shape.restype = i32
+    L.annmi_set_scan_shape.argtypes = [vp, i32, i32, i64]
     L.annmi_export.restype = i64
     L.annmi_export.argtypes = [vp, i32, vp, i64]
     L.annmi_profile.restype = i32

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -42,7 +43,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st);
 hipError_t launch_prep_queries(int kind, const float* Q, int B, int d, int kdim, void* Qscan, float* qn, float* qscale,
                                float* margin, float maxnorm, float alpha_x, const float* scale, const float* offset,
-                               hipStream_t st);
+                               hipStream_t st, uint32_t* cnt = nullptr, uint32_t* flags = nullptr,
+                               float* thr = nullptr);
 hipError_t launch_linear_index_query(const float* X, int64_t xstride, int64_t n, int d, const float* dq, int k,
                                      float* P, float* BT, float* R, float* bkey, uint32_t* bid, int64_t* d_ids,
                                      float* d_vals, uint32_t* d_flag, hipStream_t st);
@@ -197,9 +199,58 @@ struct Handle {
     float* li_vals = nullptr;
     uint32_t* li_flag = nullptr;
     int li_k = 0;
+    // Captured steps (flat modes): one executable graph per (B, k, rsel,
+    // output buffers), captured the second time that shape is seen and
+    // replayed from then on.  Workspace pointers are baked in, so the cache
+    // is dropped whenever the workspace is reallocated.
+    struct StepGraph {
+        int B, k, rsel;
+        const void* ids;
+        const void* dist;
+        hipGraphExec_t exec;  // null: seen once, not captured yet
+        bool failed;          // capture or instantiation failed: run eagerly
+        uint64_t last_use;
+    };
+    std::vector<StepGraph> graphs;
+    uint64_t graph_clock = 0;
+    // Scan kernel shape per batch size: `fixed` (annmi_set_scan_shape) for
+    // every batch, else the autotuned entry of the nearest power of two.
+    struct ScanShape {
+        int kernel = -1;  // -1 built-in rule, 0 streaming kernel, 1 wide kernel
+        int cfg = 0;      // wide configuration (kWideCfg*)
+        int64_t chunk = 0;
+        int min_batch = 0;  // fixed shapes: wide from this batch size up (0: built-in)
+    };
+    ScanShape fixed;
+    std::vector<ScanShape> tuned;  // index = log2 of the batch size
+    std::string tune_report;
 };
 
+Handle::ScanShape shape_for(const Handle* h, int B) {
+    if (!h->tuned.empty()) {
+        int lg = 0;
+        while (lg + 1 < (int)h->tuned.size() && (1 << lg) * 3 < 2 * B) ++lg;  // nearest power of two (geometric)
+        return h->tuned[(size_t)lg];
+    }
+    return h->fixed;
+}
+
+void apply_shape(const Handle::ScanShape& sh, ScanArgs& a) {
+    a.wide_cfg = sh.cfg;
+    a.chunk_max = sh.chunk;
+    if (sh.kernel == 0) a.wide_min_batch = 1 << 30;
+    else if (sh.kernel == 1) a.wide_min_batch = 1;
+    else a.wide_min_batch = sh.min_batch;
+}
+
+void drop_graphs(Handle* h) {
+    for (auto& g : h->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+}
+
 void free_ws(Handle* h) {
+    drop_graphs(h);
     dfree(h->dQ32);
     dfree(h->dQs);
     dfree(h->qn);
@@ -282,7 +333,9 @@ bool ensure_ws(Handle* h, int B, int k) {
 
 // Geometric row ranges [b_j, b_{j+1}): the first holds every row of the
 // first S0 rows, each later one grows by rho, so each adds about
-// rsel * (rho - 1) candidates on exchangeable data.
+// rsel * (rho - 1) candidates on exchangeable data.  (A longer first range
+// saves launches at small batches but costs more in the selections over its
+// candidates than it saves: measured at one query on 1M and 10M rows.)
 std::vector<int64_t> range_bounds(int64_t n, int rsel, int cap) {
     std::vector<int64_t> b{0};
     int64_t s0 = std::max<int64_t>(4096, 8 * (int64_t)rsel);
@@ -321,9 +374,9 @@ struct Profiler {
     }
 };
 
-// finalize + flag read-back + accounting, common to both step kinds.
-bool finish_step(Handle* h, int B, int k, int kk, int rsel, int64_t* ids_out, float* d_out, const CandBuf& cb,
-                 bool exact, Profiler& pr) {
+// finalize launch (enqueue only), common to both step kinds.
+bool launch_final(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_out, const CandBuf& cb, Profiler& pr,
+                  int& ev_f0, int& ev_f1) {
     hipStream_t st = h->st;
     FinalArgs fa{};
     fa.cand = cb;
@@ -342,9 +395,15 @@ bool finish_step(Handle* h, int B, int k, int kk, int rsel, int64_t* ids_out, fl
     // fp32 re-score error bound (relative, both neighbours), doubled
     const double u = 5.9604644775390625e-8;
     fa.eps_rel = 4.0 * ((h->d + 15) / 16 + 28) * u;
-    int ev_f0 = pr.ev();
+    ev_f0 = pr.ev();
     HIP_OR(launch_finalize(fa, st), false);
-    int ev_f1 = pr.ev();
+    ev_f1 = pr.ev();
+    return true;
+}
+
+// flag read-back, host synchronisation, event timing and accounting.
+bool complete_step(Handle* h, int B, bool exact, Profiler& pr, int ev_f0, int ev_f1) {
+    hipStream_t st = h->st;
     std::vector<uint32_t> flags(B), nrr(B);
     HIP_OR(hipMemcpyAsync(flags.data(), h->ovf, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
     HIP_OR(hipMemcpyAsync(nrr.data(), h->nrr, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
@@ -397,36 +456,32 @@ bool finish_step(Handle* h, int B, int k, int kk, int rsel, int64_t* ids_out, fl
     return true;
 }
 
+bool finish_step(Handle* h, int B, int k, int kk, int rsel, int64_t* ids_out, float* d_out, const CandBuf& cb,
+                 bool exact, Profiler& pr) {
+    (void)kk;
+    int ev_f0 = -1, ev_f1 = -1;
+    if (!launch_final(h, B, k, rsel, ids_out, d_out, cb, pr, ev_f0, ev_f1)) return false;
+    return complete_step(h, B, exact, pr, ev_f0, ev_f1);
+}
+
 
 bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, float* d_out);
 int32_t batch_impl(Handle* h, const float* Q, int32_t B, int32_t k, int32_t R, int64_t* ids, float* dists,
                    bool device, int32_t nprobe);
 Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params* params, int64_t src_stride);
 
-// One step over <= ws_B queries already in h->dQ32; results in d_ids/d_dist.
-bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, float* d_out) {
-    if (h->ivf) return run_step_ivf(h, B, k, R, nprobe, ids_out, d_out);
+// Enqueues one flat step (prep, geometric range scans with their filter
+// selections, finalize) on the handle's stream; no host synchronisation, so
+// it can be captured into a graph.
+bool enqueue_flat(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_out, Profiler& pr,
+                  std::vector<std::pair<int, int>>& scan_ev, std::vector<std::pair<int, int>>& sel_ev,
+                  std::vector<double>& scan_rows, std::vector<std::string>& scan_names, int& ev_f0, int& ev_f1) {
     const bool exact = h->kind == SCAN_F32;
-    const int kk = (int)std::min<int64_t>(k, h->n);
-    int rsel = exact ? kk : std::max(kk, R);
-    rsel = (int)std::min<int64_t>(rsel, h->n);
-    if (!exact && rsel > 2048) {
-        set_err("rerank R=%d exceeds the 2048-row re-score limit", rsel);
-        return false;
-    }
     hipStream_t st = h->st;
-    std::vector<std::pair<int, int>> scan_ev, sel_ev;  // (start, end) event indices
-    std::vector<double> scan_rows;
-    std::vector<std::string> scan_names;
-    int ev_used = 0;
-    Profiler pr{h, ev_used, &scan_ev, &sel_ev, &scan_rows, &scan_names, -1, -1};
     pr.ev_prep0 = pr.ev();
     HIP_OR(launch_prep_queries(h->kind, h->dQ32, B, h->d, h->kdim, h->dQs, h->qn, h->qscale, h->slack, h->maxnorm,
-                               h->alpha_x, h->sq_scale, h->sq_offset, st),
+                               h->alpha_x, h->sq_scale, h->sq_offset, st, h->cnt, h->ovf, h->thr),
            false);
-    HIP_OR(launch_fill_u32(h->cnt, 0u, B, st), false);
-    HIP_OR(launch_fill_u32(h->ovf, 0u, B, st), false);
-    HIP_OR(launch_fill_f32(h->thr, __builtin_inff(), B, st), false);
     CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
     pr.ev_prep1 = pr.ev();
     std::vector<int64_t> bounds = range_bounds(h->n, rsel, h->cap);
@@ -456,6 +511,7 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
         a.xn_lo = h->xn_lo;
         a.xn_amax = h->xn_amax;
         a.prefilter = h->prefilter;
+        apply_shape(shape_for(h, B), a);
         int e0 = pr.ev();
         const char* vname = nullptr;
         HIP_OR(launch_scan(a, st, &vname), false);
@@ -464,7 +520,87 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
         scan_rows.push_back((double)(a.row_end - a.row_begin));
         scan_names.push_back(vname ? vname : "");
     }
-    return finish_step(h, B, k, kk, rsel, ids_out, d_out, cb, exact, pr);
+    return launch_final(h, B, k, rsel, ids_out, d_out, cb, pr, ev_f0, ev_f1);
+}
+
+bool graphs_enabled() {
+    static const bool on = [] {  // ANNMI_GRAPHS=0: always launch eagerly (A/B timing)
+        const char* e = getenv("ANNMI_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+constexpr size_t kMaxGraphs = 32;
+
+// One step over <= ws_B queries already in h->dQ32; results in d_ids/d_dist.
+// Flat steps of a shape seen before replay a captured graph (no per-kernel
+// launch cost); the first step of a shape runs eagerly, the second captures.
+bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, float* d_out) {
+    if (h->ivf) return run_step_ivf(h, B, k, R, nprobe, ids_out, d_out);
+    const bool exact = h->kind == SCAN_F32;
+    const int kk = (int)std::min<int64_t>(k, h->n);
+    int rsel = exact ? kk : std::max(kk, R);
+    rsel = (int)std::min<int64_t>(rsel, h->n);
+    if (!exact && rsel > 2048) {
+        set_err("rerank R=%d exceeds the 2048-row re-score limit", rsel);
+        return false;
+    }
+    hipStream_t st = h->st;
+    std::vector<std::pair<int, int>> scan_ev, sel_ev;  // (start, end) event indices
+    std::vector<double> scan_rows;
+    std::vector<std::string> scan_names;
+    int ev_used = 0;
+    Profiler pr{h, ev_used, &scan_ev, &sel_ev, &scan_rows, &scan_names, -1, -1};
+    int ev_f0 = -1, ev_f1 = -1;
+    Handle::StepGraph* sg = nullptr;
+    if (graphs_enabled() && !h->prof) {
+        for (auto& g : h->graphs)
+            if (g.B == B && g.k == k && g.rsel == rsel && g.ids == ids_out && g.dist == d_out) sg = &g;
+        if (!sg) {
+            if (h->graphs.size() >= kMaxGraphs) {  // evict the least recently used
+                size_t lru = 0;
+                for (size_t i = 1; i < h->graphs.size(); ++i)
+                    if (h->graphs[i].last_use < h->graphs[lru].last_use) lru = i;
+                if (h->graphs[lru].exec) (void)hipGraphExecDestroy(h->graphs[lru].exec);
+                h->graphs.erase(h->graphs.begin() + (long)lru);
+            }
+            h->graphs.push_back(Handle::StepGraph{B, k, rsel, ids_out, d_out, nullptr, false, ++h->graph_clock});
+        } else {
+            sg->last_use = ++h->graph_clock;
+            if (!sg->exec && !sg->failed) {
+                // second sighting of this shape: capture the step, then replay it
+                HIP_OR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), false);
+                const bool ok = enqueue_flat(h, B, k, rsel, ids_out, d_out, pr, scan_ev, sel_ev, scan_rows, scan_names,
+                                             ev_f0, ev_f1);
+                hipGraph_t graph = nullptr;
+                const hipError_t ec = hipStreamEndCapture(st, &graph);
+                if (!ok) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return false;
+                }
+                hipGraphExec_t exec = nullptr;
+                if (ec == hipSuccess && graph &&
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+                    sg->exec = exec;
+                else
+                    sg->failed = true;
+                if (graph) (void)hipGraphDestroy(graph);
+                (void)hipGetLastError();
+                scan_ev.clear();
+                sel_ev.clear();
+                scan_rows.clear();
+                scan_names.clear();
+            }
+            if (sg->exec) {
+                HIP_OR(hipGraphLaunch(sg->exec, st), false);
+                return complete_step(h, B, exact, pr, -1, -1);
+            }
+        }
+    }
+    if (!enqueue_flat(h, B, k, rsel, ids_out, d_out, pr, scan_ev, sel_ev, scan_rows, scan_names, ev_f0, ev_f1))
+        return false;
+    return complete_step(h, B, exact, pr, ev_f0, ev_f1);
 }
 
 bool ivf_build(Handle* h);
@@ -822,6 +958,8 @@ Handle* clone_impl(Handle* src) {
     h->xn_lo = src->xn_lo;
     h->xn_amax = src->xn_amax;
     h->prefilter = src->prefilter;
+    h->fixed = src->fixed;
+    h->tuned = src->tuned;
     h->kind = src->kind;
     h->kdim = src->kdim;
     h->Xs = src->Xs;
@@ -1144,6 +1282,93 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
     return true;
 }
 
+// Autotuner: for batch sizes 1, 2, 4, ... up to max_batch, time whole query
+// steps with every candidate kernel shape (streaming kernel up to 128
+// queries; each wide configuration that fits the LDS at the default row
+// chunk, then the best one at other chunk sizes) on queries drawn from the
+// index's own rows, and keep the fastest per batch size.  Answers do not
+// depend on the shape (every shape computes the same keys), only the time.
+bool autotune_impl(Handle* h, int max_batch, int reps) {
+    if (h->ivf) {
+        set_err("autotune: IVF handles take one list-scan kernel; nothing to tune");
+        return false;
+    }
+    max_batch = std::max(1, std::min<int>(max_batch, (int)std::min<int64_t>(h->p.max_batch, 4096)));
+    reps = std::max(1, std::min(reps, 10));
+    if (!ensure_ws(h, max_batch, 100)) return false;
+    // queries: rows spread over the base (fp32 copy, padded row stride)
+    const int64_t stride_rows = std::max<int64_t>(1, h->n / max_batch);
+    HIP_OR(hipMemcpy2DAsync(h->dQ32, (size_t)h->d * 4, h->X32, (size_t)stride_rows * h->kdim32 * 4, (size_t)h->d * 4,
+                            (size_t)std::min<int64_t>(max_batch, h->n), hipMemcpyDeviceToDevice, h->st),
+           false);
+    HIP_OR(hipStreamSynchronize(h->st), false);
+    const bool prof = h->prof;
+    h->prof = false;
+    drop_graphs(h);
+    h->tuned.clear();
+    const Handle::ScanShape saved = h->fixed;
+    const int k = (int)std::min<int64_t>(100, h->n);
+    const int R = h->p.rerank > 0 ? h->p.rerank : 4 * k;
+    auto time_step = [&](int B, const Handle::ScanShape& shp, double& best) -> bool {
+        h->fixed = shp;
+        best = 1e30;
+        for (int r = 0; r <= reps; ++r) {  // the first run warms the variant up
+            const auto t0 = std::chrono::steady_clock::now();
+            if (!run_step(h, B, k, R, 0, h->d_ids, h->d_dist)) return false;
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (r > 0) best = std::min(best, ms);
+        }
+        return true;
+    };
+    std::string rep = "[";
+    std::vector<Handle::ScanShape> table;
+    bool ok = true;
+    for (int B = 1; B <= max_batch && ok; B *= 2) {
+        Handle::ScanShape def{}, best{};
+        double t_def = 0, t_best = 1e30, t = 0;
+        if (!(ok = time_step(B, def, t_def))) break;
+        best = def;
+        t_best = t_def;
+        std::vector<Handle::ScanShape> cands;
+        if (B <= 128) cands.push_back(Handle::ScanShape{0, 0, 0, 0});
+        for (int c = 1; c < kWideCfgCount; ++c)
+            if (wide_cfg_fits(h->kind, h->kdim, c)) cands.push_back(Handle::ScanShape{1, c, 0, 0});
+        for (const auto& c : cands) {
+            if (!(ok = time_step(B, c, t))) break;
+            if (t < t_best) {
+                t_best = t;
+                best = c;
+            }
+        }
+        if (ok && best.kernel == 1) {
+            const Handle::ScanShape base = best;
+            for (int64_t ch : {(int64_t)8192, (int64_t)16384, (int64_t)65536}) {
+                Handle::ScanShape c = base;
+                c.chunk = ch;
+                if (!(ok = time_step(B, c, t))) break;
+                if (t < t_best) {
+                    t_best = t;
+                    best = c;
+                }
+            }
+        }
+        if (!ok) break;
+        table.push_back(best);
+        char line[200];
+        snprintf(line, sizeof(line), "%s{\"B\": %d, \"kernel\": \"%s\", \"cfg\": %d, \"chunk\": %lld, \"ms\": %.4f, \"default_ms\": %.4f}",
+                 B == 1 ? "" : ", ", B, best.kernel == 0 ? "streaming" : (best.kernel == 1 ? "wide" : "default"),
+                 best.cfg, (long long)best.chunk, t_best, t_def);
+        rep += line;
+    }
+    h->fixed = saved;
+    h->prof = prof;
+    drop_graphs(h);
+    if (!ok) return false;
+    h->tuned = table;
+    h->tune_report = rep + "]";
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1222,6 +1447,54 @@ void* annmi_clone(void* h) {
 const char* annmi_last_error(void) { return g_err; }
 
 int64_t annmi_index_bytes(void* h) { return h ? static_cast<Handle*>(h)->bytes : 0; }
+
+int32_t annmi_autotune(void* hv, int32_t max_batch, int32_t reps, char* report, int32_t len) {
+    g_err[0] = 0;
+    Handle* h = static_cast<Handle*>(hv);
+    if (!h) {
+        set_err("null handle");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (max_batch > 0 && !autotune_impl(h, max_batch, reps)) return -1;
+    const int32_t need = (int32_t)h->tune_report.size() + 1;
+    if (report && len > 0) {
+        const int32_t m = std::min(need, len);
+        memcpy(report, h->tune_report.c_str(), (size_t)m - 1);
+        report[m - 1] = 0;
+    }
+    return need;
+}
+
+int32_t annmi_set_scan_shape(void* hv, int32_t wide_min_batch, int32_t wide_cfg, int64_t chunk_rows) {
+    g_err[0] = 0;
+    Handle* h = static_cast<Handle*>(hv);
+    if (!h) {
+        set_err("null handle");
+        return -1;
+    }
+    if (wide_min_batch < 0 || wide_cfg < 0 || wide_cfg >= kWideCfgCount || chunk_rows < 0 ||
+        (chunk_rows > 0 && chunk_rows < 2048) || (wide_cfg > 0 && !wide_cfg_fits(h->kind, h->kdim, wide_cfg))) {
+        set_err("bad scan shape: min_batch=%d cfg=%d chunk=%lld (cfg must fit the LDS, chunk 0 or >= 2048)",
+                wide_min_batch, wide_cfg, (long long)chunk_rows);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    drop_graphs(h);
+    h->tuned.clear();
+    h->tune_report.clear();
+    h->fixed = Handle::ScanShape{-1, wide_cfg, chunk_rows, wide_min_batch};
+    return 0;
+}
+
+int32_t annmi_graph_count(void* hv) {
+    if (!hv) return -1;
+    Handle* h = static_cast<Handle*>(hv);
+    std::lock_guard<std::mutex> lk(h->mu);
+    int32_t c = 0;
+    for (auto& g : h->graphs) c += g.exec ? 1 : 0;
+    return c;
+}
 
 int64_t annmi_export(void* hv, int32_t what, void* out, int64_t bytes) {
     Handle* h = static_cast<Handle*>(hv);

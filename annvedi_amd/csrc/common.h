@@ -85,7 +85,26 @@ struct ScanArgs {
     float xn_lo = -__builtin_inff();
     float xn_amax = __builtin_inff();
     int prefilter = 1;
+    // kernel shape (the autotuner's choice; 0 = the built-in rule):
+    // wide_min_batch  batches from this size up take the wide kernel
+    // wide_cfg        wide-kernel tile configuration (kWideCfg*)
+    // chunk_max       most rows per wide-kernel row chunk
+    int wide_min_batch = 0;
+    int wide_cfg = 0;
+    int64_t chunk_max = 0;
 };
+
+// Wide-kernel tile configurations: queries per workgroup tile (16*QS), rows
+// per wave step (16*RS), waves per workgroup, query fragments prefetched.
+enum annmi_wide_cfg : int {
+    kWideCfgDefault = 0,   // the built-in choice per element type and dim
+    kWideCfgQ8R4W4 = 1,    // 128 queries x 64 rows a wave, 4 waves, prefetched query fragments
+    kWideCfgQ8R2W8 = 2,    // 128 x 32, 8 waves (two a SIMD)
+    kWideCfgQ4R4W8 = 3,    // 64 x 64, 8 waves
+    kWideCfgCount = 4,
+};
+// Whether a wide configuration fits the LDS for this element type and dim.
+bool wide_cfg_fits(int kind, int kdim, int cfg);
 // Launches the scan; *variant (if not null) receives a static name of the
 // kernel instantiation launched (for per-kernel timing).
 hipError_t launch_scan(const ScanArgs& a, hipStream_t st, const char** variant = nullptr);

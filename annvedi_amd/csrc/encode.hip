@@ -193,6 +193,11 @@ struct QPrepArgs {
     float alpha_x;        // FP8 base scale
     const float* scale;   // SQ8/SQ4 per-dim scale (kdim entries for SQ4 padding)
     const float* offset;
+    // per-query step state reset here when set (saves three fill launches):
+    // candidate count 0, flags 0, filter threshold +inf
+    uint32_t* cnt;
+    uint32_t* flags;
+    float* thr;
 };
 
 __device__ float block_reduce_sum(float v, float* red) {
@@ -223,6 +228,11 @@ __global__ __launch_bounds__(256) void prep_queries(QPrepArgs a) {
     __shared__ float red[4];
     const int q = blockIdx.x;
     const float* x = a.Q + (int64_t)q * a.d;
+    if (threadIdx.x == 0 && a.cnt) {
+        a.cnt[q] = 0u;
+        a.flags[q] = 0u;
+        a.thr[q] = __builtin_inff();
+    }
     if (a.kind == SCAN_F32) {
         float* o = static_cast<float*>(a.Qscan) + (int64_t)q * a.kdim;
         float s = 0.f;
@@ -375,9 +385,10 @@ hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st) {
 
 hipError_t launch_prep_queries(int kind, const float* Q, int B, int d, int kdim, void* Qscan, float* qn, float* qscale,
                                float* margin, float maxnorm, float alpha_x, const float* scale, const float* offset,
-                               hipStream_t st) {
+                               hipStream_t st, uint32_t* cnt, uint32_t* flags, float* thr) {
     if (B <= 0) return hipSuccess;
-    QPrepArgs a{kind, Q, B, d, kdim, Qscan, qn, qscale, margin, maxnorm, alpha_x, scale, offset};
+    if ((cnt != nullptr) != (flags != nullptr) || (cnt != nullptr) != (thr != nullptr)) return hipErrorInvalidValue;
+    QPrepArgs a{kind, Q, B, d, kdim, Qscan, qn, qscale, margin, maxnorm, alpha_x, scale, offset, cnt, flags, thr};
     hipLaunchKernelGGL(prep_queries, dim3((unsigned)B), dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -733,11 +733,12 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     // row chunk: about 8 workgroups per CU over the launch, 2K..32K rows,
     // whole wave steps
     constexpr int64_t step = 16 * RS * (NW / QH);
-    static const int64_t chunk_max = [] {  // tuning: ANNMI_WIDE_CHUNK_MAX rows
+    static const int64_t chunk_env = [] {  // tuning: ANNMI_WIDE_CHUNK_MAX rows
         const char* e = getenv("ANNMI_WIDE_CHUNK_MAX");
         const long v = e ? atol(e) : 0;
         return v >= 2048 ? (int64_t)v : (int64_t)32768;
     }();
+    const int64_t chunk_max = a.chunk_max >= 2048 ? a.chunk_max : chunk_env;
     int64_t chunk = rows * qtiles / ((int64_t)ncu * 8);
     chunk = chunk < 2048 ? 2048 : (chunk > chunk_max ? chunk_max : chunk);
     chunk = (chunk + step - 1) / step * step;
@@ -757,9 +758,32 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// LDS of a wide configuration's query tile: 16*QS queries of qrow bytes plus
+// five per-query floats (the queues take what is left of the 160 KB).
+template <class P>
+bool wide_fits(int qs, int kdim) {
+    return (size_t)16 * qs * ((size_t)qrow_bytes<P>(kdim) + 20) + 8 * 1024 <= 160 * 1024;
+}
+
 // Query tile sizes (16*QS queries) per element type and dim, LDS <= 160 KB.
+// cfg selects one of the kWideCfg* shapes (the autotuner's choice); the
+// default picks per element type: 128 x 64 tiles on 4 waves for bf16 and
+// fp32, 128 x 32 tiles on 8 waves for the byte-sized codes, smaller query
+// tiles where 128 queries do not fit the LDS.
 template <class P>
 hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
+    switch (a.wide_cfg) {
+        case kWideCfgQ8R4W4:
+            if (wide_fits<P>(8, a.kdim)) return launch_wide_t<P, 8, 4, 4, 1, true>(a, st);
+            break;
+        case kWideCfgQ8R2W8:
+            if (wide_fits<P>(8, a.kdim)) return launch_wide_t<P, 8, 2, 8, 1, false>(a, st);
+            break;
+        case kWideCfgQ4R4W8:
+            if (wide_fits<P>(4, a.kdim)) return launch_wide_t<P, 4, 4, 8, 1, false>(a, st);
+            break;
+        default: break;
+    }
     const size_t row = (size_t)qrow_bytes<P>(a.kdim) + 12;
     if (128 * row <= 160 * 1024) {
         if constexpr (P::kElemBytes == 2) return launch_wide_t<P, 8, 4, 4, 1, true>(a, st);   // CFG_A
@@ -810,12 +834,26 @@ hipError_t launch_scan(const ScanArgs& a, hipStream_t st, const char** variant) 
     return e;
 }
 
+bool wide_cfg_fits(int kind, int kdim, int cfg) {
+    const int qs = cfg == kWideCfgQ4R4W8 ? 4 : 8;
+    if (cfg <= kWideCfgDefault || cfg >= kWideCfgCount) return false;
+    switch (kind) {
+        case SCAN_F32: return wide_fits<PolF32>(qs, kdim);
+        case SCAN_BF16: return wide_fits<PolBF16>(qs, kdim);
+        case SCAN_FP8: return wide_fits<PolFP8>(qs, kdim);
+        case SCAN_SQ8: return wide_fits<PolSQ8>(qs, kdim);
+        case SCAN_SQ4: return wide_fits<PolSQ4>(qs, kdim);
+    }
+    return false;
+}
+
 hipError_t launch_scan_impl(const ScanArgs& a, hipStream_t st) {
     if (a.row_end <= a.row_begin || a.B <= 0) return hipSuccess;
     // argmin scans (k-means assignment: few rows, many queries) take the
     // streaming kernel, whose per-lane minima the wide kernel has no
     // registers left for
-    if (a.B >= wide_min_batch(a.kind) && (a.row_end - a.row_begin) >= 65536 && !a.argmin) {
+    const int min_b = a.wide_min_batch > 0 ? a.wide_min_batch : wide_min_batch(a.kind);
+    if (a.B >= min_b && (a.row_end - a.row_begin) >= 65536 && !a.argmin) {
         switch (a.kind) {
             case SCAN_F32: return launch_wide<PolF32>(a, st);
             case SCAN_BF16: return launch_wide<PolBF16>(a, st);

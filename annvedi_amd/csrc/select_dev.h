@@ -58,9 +58,20 @@ __device__ uint32_t radix_select(const float* keys, int n, uint32_t rank, uint32
         __syncthreads();
         const int sft = shifts[p];
         const uint32_t wmask = (1u << widths[p]) - 1u;
-        for (int i = threadIdx.x; i < n; i += kThreads) {
-            uint32_t k = f2key(keys[i]);
-            if ((k & prefix_mask) == prefix) atomicAdd(&hist[(k >> sft) & wmask], 1u);
+        // four key loads per lane in flight
+        constexpr int U = 4;
+        for (int i0 = 0; i0 < n; i0 += kThreads * U) {
+            uint32_t k[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * kThreads + (int)threadIdx.x;
+                k[u] = i < n ? f2key(keys[i]) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * kThreads + (int)threadIdx.x;
+                if (i < n && (k[u] & prefix_mask) == prefix) atomicAdd(&hist[(k[u] >> sft) & wmask], 1u);
+            }
         }
         __syncthreads();
         int bin;

@@ -183,6 +183,32 @@ class Index:
         self._L.annmi_profile_kernels(self._h, buf, int(need))
         return json.loads(buf.value.decode())
 
+    def autotune(self, max_batch: int = 4096, reps: int = 2) -> list:
+        """Time every scan-kernel shape per batch size (1, 2, 4, ... max_batch)
+        on this index and keep the fastest (see annmi_autotune).  Returns the
+        per-batch-size report."""
+        import json
+
+        if int(max_batch) <= 0:
+            raise ValueError("max_batch must be positive")
+        need = self._L.annmi_autotune(self._h, int(max_batch), int(reps), None, 0)
+        if need < 0:
+            raise RuntimeError(_lib.last_error())
+        buf = ctypes.create_string_buffer(int(need))
+        if self._L.annmi_autotune(self._h, 0, 0, buf, int(need)) < 0:  # read the report back
+            raise RuntimeError(_lib.last_error())
+        return json.loads(buf.value.decode())
+
+    def set_scan_shape(self, wide_min_batch: int = 0, wide_cfg: int = 0, chunk_rows: int = 0):
+        """Fix the scan-kernel shape for every batch (0s: built-in rules)."""
+        if self._L.annmi_set_scan_shape(self._h, int(wide_min_batch), int(wide_cfg), int(chunk_rows)) < 0:
+            raise RuntimeError(_lib.last_error())
+
+    @property
+    def graph_count(self) -> int:
+        """Step shapes this handle replays as captured HIP graphs."""
+        return int(self._L.annmi_graph_count(self._h))
+
     @property
     def n_distances(self) -> int:
         return int(self._L.annmi_ndist(self._h))

@@ -216,6 +216,7 @@ def main():
     ap.add_argument("--rerank", type=int, default=0, help="fixed R (0: calibrate to recall >= 0.99)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain fp32 path")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in scan-kernel shapes")
     ap.add_argument("--workload", default="mixed-10m-384", choices=["mixed-10m-384", "ivf-10m-768"])
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only copy this many GiB device to device and exit (HBM counter calibration)")
@@ -268,6 +269,18 @@ def main():
                 log(f"[bench] calibrate R={R}: recall@{k} = {rc:.4f}")
                 if rc >= 0.995:
                     break
+    # scan-kernel shapes picked per batch size by the library's autotuner
+    # (before timing; its time is reported apart)
+    tune = None
+    if not args.no_autotune:
+        tune = {}
+        for name, ix in ((mode, idx),) + ((("exact", exact),) if mode != "exact" and not args.no_plain else ()):
+            t0 = time.perf_counter()
+            table = ix.autotune(max_batch=4096, reps=1)
+            tune[name] = {"seconds": round(time.perf_counter() - t0, 2),
+                          "table": [{k2: r[k2] for k2 in ("B", "kernel", "cfg", "chunk", "ms", "default_ms")}
+                                    for r in table]}
+            log(f"[bench] autotune {name}: {tune[name]}")
     # warmup, then timed steps
     wsz, tsz = sizes[:args.warmup], sizes[args.warmup:]
     run_batches(idx, Q[: int(wsz.sum())], wsz, k, R)
@@ -372,6 +385,7 @@ def main():
         "speedup_vs_plain_fp32": round(qps / plain["qps"], 3) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "autotune": tune,
     }
     print(json.dumps(res), flush=True)
 

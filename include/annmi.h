@@ -139,6 +139,38 @@ int32_t annmi_profile(void* h, int32_t enable, double* out8);
  * the length needed including the terminator, -1 on error. */
 int32_t annmi_profile_kernels(void* h, char* buf, int32_t len);
 
+/* Autotuner.  For batch sizes 1, 2, 4, ... up to max_batch (<= 4096), times
+ * whole query steps (k = 100) with every candidate scan-kernel shape on
+ * queries drawn from the index's own rows: the streaming kernel (up to 128
+ * queries) and each wide-kernel tile configuration that fits the LDS, then
+ * the best one at other row-chunk sizes; the fastest shape per batch size is
+ * kept (a batch uses the entry of the nearest power of two) until the next
+ * annmi_autotune or annmi_set_scan_shape.  Shapes change timing only: every
+ * shape computes the same candidate keys, so answers are unchanged.  reps:
+ * timed runs per candidate (best kept).  Writes a JSON report (one object
+ * per batch size: kernel, cfg, chunk, ms, default_ms) to report (len bytes);
+ * max_batch = 0 only writes the last report.  Returns the length needed
+ * including the terminator, -1 on error (IVF handles: nothing to tune).  No
+ * reference counterpart. */
+int32_t annmi_autotune(void* h, int32_t max_batch, int32_t reps, char* report, int32_t len);
+
+/* Fix the scan-kernel shape for every batch size (drops an autotuned
+ * table): batches of wide_min_batch queries and more take the wide kernel
+ * (0: built-in crossover), with tile configuration wide_cfg (0: built-in
+ * choice; 1: 128 queries x 64 rows a wave on 4 waves; 2: 128 x 32 on 8
+ * waves; 3: 64 x 64 on 8 waves) and at most chunk_rows rows per row chunk
+ * (0: built-in, else >= 2048).  Returns 0, or -1 for a configuration that does
+ * not fit the LDS at this dim.  No reference counterpart (tuning, tests). */
+int32_t annmi_set_scan_shape(void* h, int32_t wide_min_batch, int32_t wide_cfg, int64_t chunk_rows);
+
+/* Number of step shapes (batch size, k, re-score count, output buffers)
+ * whose query step this handle replays as a captured HIP graph.  A flat
+ * (non-IVF) step runs eagerly the first time its shape is seen, is captured
+ * the second time and replayed from then on (ANNMI_GRAPHS=0 disables this;
+ * steps are never captured while annmi_profile timing is on).  No reference
+ * counterpart (diagnostics).  -1 on a NULL handle. */
+int32_t annmi_graph_count(void* h);
+
 /* ------------------------------------------------------------------
  * Drop-in entry points with the exact signatures of the reference's GPU
  * contestant, so competitors/gpu_filter/algorithm.py:6-15 binds this

@@ -1,0 +1,59 @@
+"""Scan-kernel shapes: every wide-kernel tile configuration and chunk size,
+the streaming kernel, and the autotuner's table give the same answers as the
+built-in choice (a shape changes timing only)."""
+import numpy as np
+import pytest
+
+from annvedi_amd import Index, data
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+N, D = 150_000, 96
+
+
+@pytest.fixture(scope="module")
+def inputs():
+    base = data.mixture(N, D, 128, 0.3, 61, center_seed=62)
+    Q = data.mixture(40, D, 128, 0.3, 63, center_seed=62)
+    return base, Q
+
+
+@pytest.mark.parametrize("mode", ["exact", "bf16", "fp8", "sq8", "sq4"])
+def test_every_scan_shape_gives_the_same_answers(inputs, mode):
+    base, Q = inputs
+    print(f"[case] shapes mode={mode} n={N} d={D}", flush=True)
+    idx = Index(base, mode=mode, rerank=200)
+    ref_ids, ref_d = idx.search(Q, 100)
+    if mode == "exact":
+        np.testing.assert_array_equal(ref_ids, O.exact_topk_f64(base, Q, 100)[0])
+    shapes = [(1 << 30, 0, 0), (1, 0, 0), (1, 0, 4096)] + [(1, c, ch) for c in (1, 2, 3) for ch in (0, 8192)]
+    tried = 0
+    for mb, cfg, ch in shapes:
+        try:
+            idx.set_scan_shape(mb, cfg, ch)
+        except RuntimeError as e:  # configuration does not fit the LDS at this dim
+            assert "does not fit" in str(e) or "bad scan shape" in str(e)
+            continue
+        tried += 1
+        ids, dd = idx.search(Q, 100)
+        np.testing.assert_array_equal(ids, ref_ids, err_msg=f"shape {(mb, cfg, ch)}")
+        np.testing.assert_array_equal(dd, ref_d, err_msg=f"shape {(mb, cfg, ch)}")
+    assert tried >= 5
+
+
+def test_autotune_table_keeps_answers(inputs):
+    base, Q = inputs
+    print(f"[case] autotune sq8 n={N} d={D}", flush=True)
+    idx = Index(base, mode="sq8", rerank=200)
+    ref_ids, ref_d = idx.search(Q, 100)
+    rep = idx.autotune(max_batch=64, reps=1)
+    assert [r["B"] for r in rep] == [1, 2, 4, 8, 16, 32, 64]
+    for r in rep:
+        assert r["kernel"] in ("streaming", "wide", "default") and r["ms"] > 0 and r["ms"] <= r["default_ms"] * 1.0001
+    for B in (1, 3, 40):
+        ids, dd = idx.search(Q[:B], 100)
+        np.testing.assert_array_equal(ids, ref_ids[:B])
+        np.testing.assert_array_equal(dd, ref_d[:B])
+    with pytest.raises(RuntimeError):
+        idx.set_scan_shape(1, 7, 0)
