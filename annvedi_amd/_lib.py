@@ -29,7 +29,7 @@ EXPORTS = [
     "annmi_batch_query_ex", "annmi_query_linear_index",
     "annmi_ndist", "annmi_last_batch_stats", "annmi_free", "annmi_clone", "annmi_last_error", "annmi_index_bytes",
     "annmi_profile", "annmi_profile_kernels", "annmi_export", "annmi_graph_count",
-    "annmi_autotune", "annmi_set_scan_shape",
+    "annmi_autotune", "annmi_set_scan_shape", "annmi_save", "annmi_load",
     "gf_create", "gf_query", "gf_ndist", "gf_free", "gf_last_error",
 ]
 
@@ -108,6 +108,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.annmi_autotune.argtypes = [vp, i32, i32, ctypes.c_char_p, i32]
     L.annmi_set_scan_shape.restype = i32
     L.annmi_set_scan_shape.argtypes = [vp, i32, i32, i64]
+    L.annmi_save.restype = i64
+    L.annmi_save.argtypes = [vp, vp, i64]
+    L.annmi_load.restype = vp
+    L.annmi_load.argtypes = [vp, i64, i32, ctypes.POINTER(Params), vp, i64]
     L.annmi_export.restype = i64
     L.annmi_export.argtypes = [vp, i32, vp, i64]
     L.annmi_profile.restype = i32

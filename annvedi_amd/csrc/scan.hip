@@ -524,7 +524,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
         // ~1000x the fp32 rounding of the test; the exact test decides.
         // th = -inf (padding) gives NaN: never passes; th = +inf: -inf.
         const float M = a.xn_amax + fabsf(qv) + fabsf(th);
-        bnd_s[m] = (qv - th) - M * 0x1p-14f;
+        bnd_s[m] = 0.5f * ((qv - th) - M * 0x1p-14f);  // halved: the test scales by 1/2
         // global form (prefilter 2): xn >= xn_lo for every row, so
         // dot >= ((xn_lo + qv) - th - margin) / (2 qs) is necessary
         const float sc = 2.0f * qs;
@@ -658,15 +658,20 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
         turn(0, std::true_type{});
         for (int c0 = D; c0 < npad; c0 += D) turn(c0, std::false_type{});
 
-        // epilogue.  Prefilter per (query, lane): the largest 2*qs*dot - xn
-        // over the lane's RS rows against the query's bound (or, when all
-        // row norms are nearly equal, the largest dot against one bound per
-        // query); only lanes and query groups that may pass evaluate the
-        // exact test and append.
+        // epilogue.  Prefilter per (query, lane): a pair can pass
+        // dist = xn + qv - 2*qs*dot <= th only if qs*dot - xn/2 reaches the
+        // query's bound (half of the margin-widened qv - th).  Over the lane's
+        // RS rows, qs*max(dot) - min(xn)/2 bounds every row's value from above
+        // (qs > 0, fma rounding is monotone), so testing it per (query, lane)
+        // is still necessary for every row; when all row norms are nearly
+        // equal, max(dot) against one bound per query instead.  Only lanes
+        // and query groups that may pass evaluate the exact test and append,
+        // and a tile where no lane of the wave may pass skips that part.
+        uint32_t pend = 0;  // bit q: query group q may pass in this lane
+        if (a.prefilter == 2) {
 #pragma unroll
-        for (int q = 0; q < QS; ++q) {
-            bool maybe = false;
-            if (a.prefilter == 2) {
+            for (int q = 0; q < QS; ++q) {
+                bool maybe = false;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     El mx = acc[q][0][i];
@@ -674,18 +679,33 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                     for (int r = 1; r < RS; ++r) mx = acc[q][r][i] > mx ? acc[q][r][i] : mx;
                     maybe |= mx >= bndg_s[qbase + 16 * q + 4 * g + i];
                 }
-            } else {
+                pend |= maybe ? 1u << q : 0u;
+            }
+        } else {
+            float hx = xn[0];
+#pragma unroll
+            for (int r = 1; r < RS; ++r) hx = fminf(hx, xn[r]);
+            hx *= 0.5f;
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                const int m0 = qbase + 16 * q + 4 * g;
+                const f32x4 qs4 = *reinterpret_cast<const f32x4*>(qsc_s + m0);
+                const f32x4 bn4 = *reinterpret_cast<const f32x4*>(bnd_s + m0);
+                bool maybe = false;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int m = qbase + 16 * q + 4 * g + i;
-                    const float sc = 2.0f * qsc_s[m];
-                    float mx = fmaf(sc, (float)acc[q][0][i], -xn[0]);
+                    El mx = acc[q][0][i];
 #pragma unroll
-                    for (int r = 1; r < RS; ++r) mx = fmaxf(mx, fmaf(sc, (float)acc[q][r][i], -xn[r]));
-                    maybe |= mx >= bnd_s[m];
+                    for (int r = 1; r < RS; ++r) mx = acc[q][r][i] > mx ? acc[q][r][i] : mx;
+                    maybe |= fmaf(qs4[i], (float)mx, -hx) >= bn4[i];
                 }
+                pend |= maybe ? 1u << q : 0u;
             }
-            if (maybe) {
+        }
+        if (__ballot(pend != 0u) != 0) {
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                if (!((pend >> q) & 1u)) continue;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int m = qbase + 16 * q + 4 * g + i;

@@ -16,7 +16,9 @@ from . import _lib
 
 class Index:
     def __init__(self, base, mode: str = "exact", rerank: int = 0, sq_variant: int = 0, max_batch: int = 4096,
-                 nlist: int = 0, nprobe: int = 0, kmeans_iters: int = 0, kmeans_seed: int = 0):
+                 nlist: int = 0, nprobe: int = 0, kmeans_iters: int = 0, kmeans_seed: int = 0, saved=None):
+        """saved: bytes of Index.save_bytes() (or a path via Index.load) for
+        the same base: recreates that index without re-encoding or k-means."""
         L = _lib.load()
         if mode not in _lib.MODES:
             raise ValueError(f"unknown mode {mode!r}; one of {sorted(_lib.MODES)}")
@@ -54,11 +56,36 @@ class Index:
             n, d = arr.shape
             ptr = arr.ctypes.data_as(ctypes.c_void_p)
         self.n, self.d = int(n), int(d)
-        h = L.annmi_create(ptr, self.n, self.d, ctypes.byref(p))
+        if saved is not None:
+            blob = np.frombuffer(saved, dtype=np.uint8)
+            h = L.annmi_load(ptr, self.n, self.d, ctypes.byref(p), blob.ctypes.data_as(ctypes.c_void_p), blob.size)
+        else:
+            h = L.annmi_create(ptr, self.n, self.d, ctypes.byref(p))
         if not h:
             raise RuntimeError(_lib.last_error())
         self._h = ctypes.c_void_p(h)
         self._L = L
+
+    def save_bytes(self) -> bytes:
+        """The index as a byte image (annmi_save): derived arrays only; the
+        fp32 base is passed again to load it."""
+        need = self._L.annmi_save(self._h, None, 0)
+        if need < 0:
+            raise RuntimeError(_lib.last_error())
+        buf = np.empty(int(need), dtype=np.uint8)
+        if self._L.annmi_save(self._h, buf.ctypes.data_as(ctypes.c_void_p), int(need)) < 0:
+            raise RuntimeError(_lib.last_error())
+        return buf.tobytes()
+
+    def save(self, path: str) -> None:
+        with open(path, "wb") as f:
+            f.write(self.save_bytes())
+
+    @classmethod
+    def load(cls, path: str, base, mode: str, **kw) -> "Index":
+        """An index written by save() over the same base."""
+        with open(path, "rb") as f:
+            return cls(base, mode=mode, saved=f.read(), **kw)
 
     def clone(self) -> "Index":
         """Another handle on the same device-resident index with its own HIP

@@ -135,15 +135,36 @@ int64_t annmi_export(void* h, int32_t what, void* out, int64_t bytes);
 int32_t annmi_profile(void* h, int32_t enable, double* out8);
 
 /* The same counters split by scan kernel instantiation, as a JSON array of
- * {kernel, launches, ms, bytes, flops} written to buf (len bytes).  Returns
- * the length needed including the terminator, -1 on error. */
+ * {kernel, launches, ms, bytes, flops, launch_ms[], launch_bytes[],
+ * launch_flops[]} (the per-launch lists hold the first 4096 launches;
+ * bytes and flops are algorithmic) written to buf (len bytes).  Returns the
+ * length needed including the terminator, -1 on error. */
 int32_t annmi_profile_kernels(void* h, char* buf, int32_t len);
+
+/* Save an index: writes a self-describing byte image (header, SQ scale and
+ * offset, scan rows -- SQ4 nibbles in the submission/src/hnsw.cpp:170-182
+ * layout --, row norms, and for IVF the list permutation, list offsets and
+ * centroids) to buf when bytes is large enough.  The fp32 base is not part
+ * of it.  Returns the size needed, -1 on error.  No reference counterpart:
+ * the reference rebuilds every index in fit (README.md:12-13). */
+int64_t annmi_save(void* h, void* buf, int64_t bytes);
+
+/* Recreate a saved index over the same n x d float32 base (host or device,
+ * as in annmi_create) without the IVF k-means build or the encoders.  params
+ * (NULL: defaults for the saved mode) must name the saved mode; the saved
+ * encoder variant and list count are used.  Returns NULL (annmi_last_error)
+ * when the image is truncated, corrupt, of another mode or shape, or of a
+ * layout this library does not use.  No reference counterpart. */
+void* annmi_load(const float* base, int64_t n, int32_t d, const annmi_params* params, const void* saved,
+                 int64_t bytes);
 
 /* Autotuner.  For batch sizes 1, 2, 4, ... up to max_batch (<= 4096), times
  * whole query steps (k = 100) with every candidate scan-kernel shape on
  * queries drawn from the index's own rows: the streaming kernel (up to 128
- * queries) and each wide-kernel tile configuration that fits the LDS, then
- * the best one at other row-chunk sizes; the fastest shape per batch size is
+ * queries), each wide-kernel tile configuration that fits the LDS, the
+ * built-in one at other row-chunk sizes, then the best one at other chunk
+ * sizes.  A shape replaces the built-in choice only when at least 2% faster
+ * (eager steps, no graph capture while tuning); the result per batch size is
  * kept (a batch uses the entry of the nearest power of two) until the next
  * annmi_autotune or annmi_set_scan_shape.  Shapes change timing only: every
  * shape computes the same candidate keys, so answers are unchanged.  reps:

@@ -12,7 +12,7 @@ EXT      := $(shell python3 -c "import sysconfig;print(sysconfig.get_config_var(
 CXX      ?= g++
 REFFLAGS := -O3 -ffast-math -march=x86-64-v4 -mtune=generic -pthread -fPIC -shared -fvisibility=hidden
 
-all: $(OUT)/linear_ann_cpp$(EXT) $(OUT)/ivf_lsh_cpp$(EXT)
+all: $(OUT)/linear_ann_cpp$(EXT) $(OUT)/ivf_lsh_cpp$(EXT) $(OUT)/hnsw_cpp$(EXT)
 
 # src/linear_index.cpp + src/simd.h + src/thread_pool.h  (setup.py:5-12)
 $(OUT)/linear_ann_cpp$(EXT): $(REF)/src/linear_index.cpp $(REF)/src/simd.h
@@ -25,3 +25,10 @@ $(OUT)/ivf_lsh_cpp$(EXT): $(REF)/competitors/ivf_lsh/src/lsh_index_optimized.cpp
 	$(CXX) -std=c++14 $(REFFLAGS) -I$(REF)/competitors/ivf_lsh/src -I$(PY_INC) -I$(PB_INC) $< -o $@
 
 .PHONY: all
+
+# submission/src/hnsw.cpp with ITS OWN simd.h (submission/setup.py:4-13): the
+# reference's HNSW (per-dim SQ8, heuristic neighbours, reorder), timed beside
+# the GPU path as a CPU baseline
+$(OUT)/hnsw_cpp$(EXT): $(REF)/submission/src/hnsw.cpp $(REF)/submission/src/simd.h
+	@mkdir -p $(OUT)
+	$(CXX) -std=c++17 $(REFFLAGS) -I$(REF)/submission/src -I$(PY_INC) -I$(PB_INC) $< -o $@

@@ -79,3 +79,20 @@ def test_no_device_create_raises_runtime_error():
     a = Algorithm()
     with pytest.raises(RuntimeError, match="device"):
         a.fit(np.random.rand(50, 16).astype(np.float32))
+
+
+def test_saved_image_header_checked_before_any_device_call():
+    """annmi_load refuses a truncated or foreign image with a message (no
+    device needed to get there)."""
+    L = _lib.load()
+    p = _lib.Params()
+    L.annmi_default_params(ctypes.byref(p), _lib.ANNMI_SQ8)
+    base = np.zeros((10, 16), dtype=np.float32)
+    junk = np.zeros(64, dtype=np.uint8)
+    h = L.annmi_load(base.ctypes.data_as(ctypes.c_void_p), 10, 16, ctypes.byref(p),
+                     junk.ctypes.data_as(ctypes.c_void_p), junk.size)
+    assert not h and "too short" in _lib.last_error()
+    junk = np.zeros(4096, dtype=np.uint8)
+    h = L.annmi_load(base.ctypes.data_as(ctypes.c_void_p), 10, 16, ctypes.byref(p),
+                     junk.ctypes.data_as(ctypes.c_void_p), junk.size)
+    assert not h and "not an annmi index" in _lib.last_error()
