@@ -36,7 +36,7 @@
 
 hipError_t launch_encode(int kind, const float* src, int64_t sstride, int64_t n, int d, int kdim, void* dst, float* xn,
                          uint32_t* maxkey, float alpha, const float* scale, const float* offset, int variant,
-                         hipStream_t st, const uint32_t* gather = nullptr);
+                         hipStream_t st, const uint32_t* gather = nullptr, int tile_cb = 0);
 hipError_t launch_dim_minmax(const float* src, int64_t sstride, int64_t n, int d, uint32_t* mn, uint32_t* mx,
                              hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
@@ -115,6 +115,20 @@ int64_t scan_q_bytes(int kind, int kdim) {
     }
 }
 
+// Chunk bytes of the tiled layout for a scan element type.
+int tile_cb_of(int kind) { return (int)scan_row_bytes(kind, chunk_of(kind)); }
+
+// Host conversions between the row-major and the tiled layout (n rows of
+// `stride` bytes; the tiled buffer holds whole 16-row tiles).
+void untile_rows(const uint8_t* tiled, uint8_t* rows, int64_t n, int64_t stride, int cb) {
+    for (int64_t r = 0; r < n; ++r)
+        for (int64_t b = 0; b < stride; b += cb) memcpy(rows + r * stride + b, tiled + tiled_off(r, b, stride, cb), (size_t)cb);
+}
+void tile_rows(const uint8_t* rows, uint8_t* tiled, int64_t n, int64_t stride, int cb) {
+    for (int64_t r = 0; r < n; ++r)
+        for (int64_t b = 0; b < stride; b += cb) memcpy(tiled + tiled_off(r, b, stride, cb), rows + r * stride + b, (size_t)cb);
+}
+
 struct Handle {
     annmi_params p{};
     int64_t n = 0;
@@ -131,6 +145,7 @@ struct Handle {
     int kind = 0;
     int kdim = 0;
     void* Xs = nullptr;  // scan base
+    int tiled = 0;       // Xs in the tiled layout (quantized flat modes)
     bool own_xs = false;
     int64_t xs_stride = 0;
     float* xn = nullptr;
@@ -338,12 +353,27 @@ bool ensure_ws(Handle* h, int B, int k) {
 // rsel * (rho - 1) candidates on exchangeable data.  (A longer first range
 // saves launches at small batches but costs more in the selections over its
 // candidates than it saves: measured at one query on 1M and 10M rows.)
-std::vector<int64_t> range_bounds(int64_t n, int rsel, int cap) {
+std::vector<int64_t> range_bounds(int64_t n, int rsel, int cap, int B) {
+    // Large batches append rsel * (rho - 1) candidates per query per range
+    // (each an LDS and a global atomic), small ones pay a launch and a
+    // selection per range: large batches take a short first range and a
+    // small growth factor, small batches few ranges.  ANNMI_RANGE_S0 /
+    // ANNMI_RANGE_RHO / ANNMI_RANGE_BATCH override the first range, the
+    // factor and the batch size from which the large-batch rule applies
+    // (tuning).
+    static const int64_t env_s0 = getenv("ANNMI_RANGE_S0") ? atoll(getenv("ANNMI_RANGE_S0")) : 0;
+    static const int64_t env_rho = getenv("ANNMI_RANGE_RHO") ? atoll(getenv("ANNMI_RANGE_RHO")) : 0;
+    static const int64_t env_b = getenv("ANNMI_RANGE_BATCH") ? atoll(getenv("ANNMI_RANGE_BATCH")) : 0;
+    const bool large = B >= (env_b > 0 ? env_b : 16);
     std::vector<int64_t> b{0};
-    int64_t s0 = std::max<int64_t>(4096, 8 * (int64_t)rsel);
-    s0 = std::min<int64_t>(s0, cap / 2);
-    s0 = std::max<int64_t>(s0, rsel);
+    int64_t s0 = env_s0 > 0 ? env_s0 : (large ? std::max<int64_t>(1024, 8 * (int64_t)rsel)
+                                              : std::max<int64_t>(4096, 8 * (int64_t)rsel));
+    s0 = (s0 + 15) / 16 * 16;  // ranges start on 16-row tiles of the tiled layout
+    s0 = std::min<int64_t>(s0, cap / 2 / 16 * 16);
+    s0 = std::max<int64_t>(s0, (rsel + 15) / 16 * 16);
     int64_t rho = std::max<int64_t>(4, std::min<int64_t>(32, cap / (8 * (int64_t)rsel)));
+    if (large) rho = std::min<int64_t>(rho, 4);
+    if (env_rho >= 2) rho = env_rho;
     int64_t cur = std::min<int64_t>(n, s0);
     b.push_back(cur);
     while (cur < n) {
@@ -492,7 +522,7 @@ bool enqueue_flat(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
            false);
     CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
     pr.ev_prep1 = pr.ev();
-    std::vector<int64_t> bounds = range_bounds(h->n, rsel, h->cap);
+    std::vector<int64_t> bounds = range_bounds(h->n, rsel, h->cap, B);
     for (size_t j = 0; j + 1 < bounds.size(); ++j) {
         if (j > 0) {
             SelectArgs sa{cb, B, rsel, h->slack, h->thr, h->ovf};
@@ -519,6 +549,7 @@ bool enqueue_flat(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
         a.xn_lo = h->xn_lo;
         a.xn_amax = h->xn_amax;
         a.prefilter = h->prefilter;
+        a.tiled = h->tiled;
         apply_shape(shape_for(h, B), a);
         int e0 = pr.ev();
         const char* vname = nullptr;
@@ -659,10 +690,12 @@ bool setup_mode(Handle* h) {
     }
     h->kdim = round_up(d, chunk_of(h->kind));
     h->xs_stride = scan_row_bytes(h->kind, h->kdim);
-    HIP_OR(hipMalloc(&h->Xs, n * h->xs_stride), false);
+    h->tiled = h->ivf ? 0 : 1;  // IVF lists start at any row: row-major
+    const int64_t n_alloc = h->tiled ? (n + 15) / 16 * 16 : n;
+    HIP_OR(hipMalloc(&h->Xs, n_alloc * h->xs_stride), false);
     h->own_xs = true;
     HIP_OR(hipMalloc(&h->xn, n * sizeof(float)), false);
-    h->bytes += n * h->xs_stride + n * 4;
+    h->bytes += n_alloc * h->xs_stride + n * 4;
     float* sc = nullptr;
     float* of = nullptr;
     if (h->kind == SCAN_FP8 || h->kind == SCAN_SQ8 || h->kind == SCAN_SQ4) {
@@ -716,7 +749,7 @@ bool setup_mode(Handle* h) {
         }
     }
     HIP_OR(launch_encode(h->kind, h->X32, h->kdim32, n, d, h->kdim, h->Xs, h->xn, nullptr, h->alpha_x, sc, of,
-                         h->p.sq_variant, st, h->ivf ? h->perm : nullptr),
+                         h->p.sq_variant, st, h->ivf ? h->perm : nullptr, h->tiled ? tile_cb_of(h->kind) : 0),
            false);
     HIP_OR(hipStreamSynchronize(st), false);
     if (h->ivf) {
@@ -953,9 +986,26 @@ bool load_mode(Handle* h, const uint8_t* blob) {
     h->kdim = hd.kdim;
     h->xs_stride = hd.row_stride;
     h->alpha_x = hd.alpha_x;
-    if (!up(&h->Xs, 2, n * h->xs_stride) || !up((void**)&h->xn, 3, n * 4)) return false;
+    h->tiled = h->ivf ? 0 : 1;
+    if (h->tiled) {
+        // the image holds row-major rows: tile them on the host
+        if (sec(2) < n * h->xs_stride) {
+            set_err("saved index: section 2 holds %lld bytes, %lld needed", (long long)sec(2),
+                    (long long)(n * h->xs_stride));
+            return false;
+        }
+        const int64_t n_alloc = (n + 15) / 16 * 16;
+        std::vector<uint8_t> t((size_t)(n_alloc * h->xs_stride), 0);
+        tile_rows(blob + hd.off[2], t.data(), n, h->xs_stride, tile_cb_of(h->kind));
+        HIP_OR(hipMalloc(&h->Xs, t.size()), false);
+        HIP_OR(hipMemcpy(h->Xs, t.data(), t.size(), hipMemcpyHostToDevice), false);
+        if (!up((void**)&h->xn, 3, n * 4)) return false;
+        h->bytes += n_alloc * h->xs_stride + n * 4;
+    } else {
+        if (!up(&h->Xs, 2, n * h->xs_stride) || !up((void**)&h->xn, 3, n * 4)) return false;
+        h->bytes += n * h->xs_stride + n * 4;
+    }
     h->own_xs = true;
-    h->bytes += n * h->xs_stride + n * 4;
     if (h->kind == SCAN_SQ8 || h->kind == SCAN_SQ4) {
         if (!up((void**)&h->sq_scale, 0, (int64_t)h->kdim * 4) || !up((void**)&h->sq_offset, 1, (int64_t)h->kdim * 4))
             return false;
@@ -1098,6 +1148,7 @@ Handle* clone_impl(Handle* src) {
     h->kdim = src->kdim;
     h->Xs = src->Xs;
     h->xs_stride = src->xs_stride;
+    h->tiled = src->tiled;
     h->xn = src->xn;
     h->alpha_x = src->alpha_x;
     h->sq_scale = src->sq_scale;
@@ -1610,6 +1661,13 @@ int64_t annmi_save(void* hv, void* buf, int64_t bytes) {
             memcpy(out + hd.off[5], h->list_off.data(), h->list_off.size() * 8);
             continue;
         }
+        if (i == 2 && h->tiled) {  // rows leave the tiled layout: the image is row-major
+            const int64_t n_alloc = (h->n + 15) / 16 * 16;
+            std::vector<uint8_t> t((size_t)(n_alloc * h->xs_stride));
+            HIP_OR(hipMemcpy(t.data(), h->Xs, t.size(), hipMemcpyDeviceToHost), -1);
+            untile_rows(t.data(), out + hd.off[2], h->n, h->xs_stride, tile_cb_of(h->kind));
+            continue;
+        }
         int64_t m = 0;  // bytes of the array (sections are padded)
         switch (i) {
             case 0: case 1: m = (int64_t)h->kdim * 4; break;
@@ -1746,6 +1804,13 @@ int64_t annmi_export(void* hv, int32_t what, void* out, int64_t bytes) {
         default: set_err("unknown export %d", what); return -1;
     }
     if (!out || bytes < need) return need;
+    if (what == 0 && h->tiled) {  // row-major rows, whatever the device layout
+        const int64_t n_alloc = (h->n + 15) / 16 * 16;
+        std::vector<uint8_t> t((size_t)(n_alloc * h->xs_stride));
+        HIP_OR(hipMemcpy(t.data(), h->Xs, t.size(), hipMemcpyDeviceToHost), -1);
+        untile_rows(t.data(), static_cast<uint8_t*>(out), h->n, h->xs_stride, tile_cb_of(h->kind));
+        return need;
+    }
     HIP_OR(hipMemcpy(out, src, (size_t)need, hipMemcpyDeviceToHost), -1);
     return need;
 }

@@ -29,6 +29,16 @@ __host__ __device__ inline float next_up(float f) {
     return key2f(f2key(f) + 1u);
 }
 
+// Tiled scan-row layout (the quantized scan copies): rows are grouped in
+// tiles of 16, and a tile is stored chunk-major: the cb bytes of chunk c of
+// its 16 rows are contiguous (16 * cb bytes), then chunk c + 1.  One 16-byte
+// load per lane of a wave then reads one contiguous 1 KB block (16 rows x
+// 64 bytes) instead of 16 half cache lines 1 row apart.  A partial last tile
+// occupies a whole tile of memory.  Byte offset of byte b of row `row`:
+__host__ __device__ inline int64_t tiled_off(int64_t row, int64_t b, int64_t row_bytes, int cb) {
+    return (row >> 4) * 16 * row_bytes + (b / cb) * 16 * cb + (row & 15) * cb + (b % cb);
+}
+
 // Scan modes (element type of the candidate-generation pass).
 enum annmi_scan_kind : int {
     SCAN_F32 = 0,
@@ -92,6 +102,9 @@ struct ScanArgs {
     int wide_min_batch = 0;
     int wide_cfg = 0;
     int64_t chunk_max = 0;
+    // X in the tiled layout (tiled_off, chunk bytes of the element type)
+    // instead of row-major
+    int tiled = 0;
 };
 
 // Wide-kernel tile configurations: queries per workgroup tile (16*QS), rows

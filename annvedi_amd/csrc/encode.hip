@@ -45,8 +45,14 @@ __global__ __launch_bounds__(256) void enc_f32(const float* src, int64_t sstride
     }
 }
 
+// Destination of element j of row `row` (element size es bytes): row-major,
+// or tiled (cb > 0: chunk bytes of the tiled layout).
+__device__ inline int64_t dst_elem(int64_t row, int j, int kdim, int es, int cb) {
+    return cb > 0 ? tiled_off(row, (int64_t)j * es, (int64_t)kdim * es, cb) / es : row * kdim + j;
+}
+
 __global__ __launch_bounds__(256) void enc_bf16(const float* src, int64_t sstride, int64_t n, int d, int kdim,
-                                                __bf16* dst, float* xn) {
+                                                __bf16* dst, float* xn, int cb) {
     const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
     const int l = threadIdx.x & 15;
     if (row >= n) return;
@@ -54,7 +60,7 @@ __global__ __launch_bounds__(256) void enc_bf16(const float* src, int64_t sstrid
     float s = 0.f;
     for (int j = l; j < kdim; j += 16) {
         __bf16 b = (__bf16)(j < d ? p[j] : 0.f);
-        dst[row * kdim + j] = b;
+        dst[dst_elem(row, j, kdim, 2, cb)] = b;
         float v = (float)b;
         s = fmaf(v, v, s);
     }
@@ -73,7 +79,7 @@ __device__ inline float from_e4m3(uint8_t b) {
 }
 
 __global__ __launch_bounds__(256) void enc_fp8(const float* src, int64_t sstride, int64_t n, int d, int kdim,
-                                               float alpha, uint8_t* dst, float* xn) {
+                                               float alpha, uint8_t* dst, float* xn, int cb) {
     const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
     const int l = threadIdx.x & 15;
     if (row >= n) return;
@@ -81,7 +87,7 @@ __global__ __launch_bounds__(256) void enc_fp8(const float* src, int64_t sstride
     float s = 0.f;
     for (int j = l; j < kdim; j += 16) {
         uint8_t b = to_e4m3(j < d ? p[j] * alpha : 0.f);
-        dst[row * kdim + j] = b;
+        dst[dst_elem(row, j, kdim, 1, cb)] = b;
         float v = from_e4m3(b);
         s = fmaf(v, v, s);
     }
@@ -95,7 +101,7 @@ __device__ inline int sq_round(float t, int variant) {
 
 __global__ __launch_bounds__(256) void enc_sq8(const float* src, int64_t sstride, int64_t n, int d, int kdim,
                                                const float* scale, const float* offset, int variant, uint8_t* dst,
-                                               float* xn, const uint32_t* gather) {
+                                               float* xn, const uint32_t* gather, int cb) {
     const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
     const int l = threadIdx.x & 15;
     if (row >= n) return;
@@ -109,7 +115,7 @@ __global__ __launch_bounds__(256) void enc_sq8(const float* src, int64_t sstride
             float v = fmaf((float)c, scale[j], offset[j]);  // reconstructed value
             s = fmaf(v, v, s);
         }
-        dst[row * kdim + j] = c;
+        dst[dst_elem(row, j, kdim, 1, cb)] = c;
     }
     s = warp16_sum(s);
     if (l == 0) xn[row] = s;
@@ -118,7 +124,8 @@ __global__ __launch_bounds__(256) void enc_sq8(const float* src, int64_t sstride
 // SQ4: one thread per output byte (two dims).  kdim is a multiple of 64;
 // dims past d are code 0 with scale 0.
 __global__ __launch_bounds__(256) void enc_sq4(const float* src, int64_t sstride, int64_t n, int d, int kdim,
-                                               const float* scale, const float* offset, uint8_t* dst, float* xn) {
+                                               const float* scale, const float* offset, uint8_t* dst, float* xn,
+                                               int cb) {
     const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
     const int l = threadIdx.x & 15;
     if (row >= n) return;
@@ -141,7 +148,7 @@ __global__ __launch_bounds__(256) void enc_sq4(const float* src, int64_t sstride
             float f = fmaf((float)nhi, scale[dhi], offset[dhi]);
             s = fmaf(f, f, s);
         }
-        dst[row * stride + b] = (uint8_t)(nlo | (nhi << 4));
+        dst[cb > 0 ? tiled_off(row, b, stride, cb) : row * stride + b] = (uint8_t)(nlo | (nhi << 4));
     }
     s = warp16_sum(s);
     if (l == 0) xn[row] = s;
@@ -331,8 +338,9 @@ __global__ __launch_bounds__(256) void prep_queries(QPrepArgs a) {
 
 hipError_t launch_encode(int kind, const float* src, int64_t sstride, int64_t n, int d, int kdim, void* dst, float* xn,
                          uint32_t* maxkey, float alpha, const float* scale, const float* offset, int variant,
-                         hipStream_t st, const uint32_t* gather) {
+                         hipStream_t st, const uint32_t* gather, int tile_cb) {
     if (gather && kind != SCAN_SQ8) return hipErrorInvalidValue;
+    if (tile_cb < 0 || (tile_cb > 0 && (kind == SCAN_F32 || gather))) return hipErrorInvalidValue;
     if (n <= 0) return hipSuccess;
     dim3 grid((unsigned)((n + 15) / 16));
     switch (kind) {
@@ -342,19 +350,19 @@ hipError_t launch_encode(int kind, const float* src, int64_t sstride, int64_t n,
             break;
         case SCAN_BF16:
             hipLaunchKernelGGL(enc_bf16, grid, dim3(256), 0, st, src, sstride, n, d, kdim, static_cast<__bf16*>(dst),
-                               xn);
+                               xn, tile_cb);
             break;
         case SCAN_FP8:
             hipLaunchKernelGGL(enc_fp8, grid, dim3(256), 0, st, src, sstride, n, d, kdim, alpha,
-                               static_cast<uint8_t*>(dst), xn);
+                               static_cast<uint8_t*>(dst), xn, tile_cb);
             break;
         case SCAN_SQ8:
             hipLaunchKernelGGL(enc_sq8, grid, dim3(256), 0, st, src, sstride, n, d, kdim, scale, offset, variant,
-                               static_cast<uint8_t*>(dst), xn, gather);
+                               static_cast<uint8_t*>(dst), xn, gather, tile_cb);
             break;
         case SCAN_SQ4:
             hipLaunchKernelGGL(enc_sq4, grid, dim3(256), 0, st, src, sstride, n, d, kdim, scale, offset,
-                               static_cast<uint8_t*>(dst), xn);
+                               static_cast<uint8_t*>(dst), xn, tile_cb);
             break;
         default: return hipErrorInvalidValue;
     }

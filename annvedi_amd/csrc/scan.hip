@@ -175,11 +175,13 @@ __device__ inline i32x4 to_frag(const typename RawOf<P>::T& v) {
     }
 }
 
-// Fragment of row `row` covering dims [dim0, dim0 + kDimsPerLoad).
+// Fragment of row `row` covering dims [dim0, dim0 + kDimsPerLoad), in the
+// row-major or the tiled layout (tiled_off).
 template <class P>
-__device__ inline i32x4 load_row_frag(const uint8_t* X, int64_t xstride, int64_t row, int dim0) {
+__device__ inline i32x4 load_row_frag(const uint8_t* X, int64_t xstride, int64_t row, int dim0, int tiled) {
     const int64_t off = P::kElemBytes == 0 ? (dim0 >> 1) : (int64_t)dim0 * P::kElemBytes;
-    return to_frag<P>(load_raw<P>(X + row * xstride + off));
+    const int64_t at = tiled ? tiled_off(row, off, xstride, chunk_bytes<P>()) : row * xstride + off;
+    return to_frag<P>(load_raw<P>(X + at));
 }
 
 // Query tile in LDS: 16*QW queries x kdim elements, stored in the scan
@@ -270,7 +272,8 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
         int c = 0;
         for (; c + U <= nchunk; c += U) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) buf[u] = load_row_frag<P>(X, a.xstride, lrow, (c + u) * P::kChunk + lane_dim);
+            for (int u = 0; u < U; ++u)
+                buf[u] = load_row_frag<P>(X, a.xstride, lrow, (c + u) * P::kChunk + lane_dim, a.tiled);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -283,7 +286,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
             }
         }
         for (; c < nchunk; ++c) {
-            i32x4 b = load_row_frag<P>(X, a.xstride, lrow, c * P::kChunk + lane_dim);
+            i32x4 b = load_row_frag<P>(X, a.xstride, lrow, c * P::kChunk + lane_dim, a.tiled);
 #pragma unroll
             for (int w = 0; w < QW; ++w) {
                 const int qoff = (16 * w + n) * ldq;
@@ -567,13 +570,23 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     typedef typename RawOf<P>::T Raw;
     Raw ring[D][RS];
     uint32_t cur[RS], nxt[RS];
-    const uint8_t* xbase = X + r_begin * a.xstride;
+    // Tiled layout: chunk c of a 16-row tile lies 16 * CB bytes after chunk
+    // c - 1, and lane n's row is the tile's row n (row offsets from the tile
+    // holding the chunk's first row).
+    const int64_t cstep = a.tiled ? 16 * CB : CB;
+    const uint8_t* xbase = a.tiled ? X + (r_begin >> 4) * 16 * a.xstride : X + r_begin * a.xstride;
     auto tile_offs = [&](int64_t t, uint32_t (&o)[RS]) {
 #pragma unroll
         for (int r = 0; r < RS; ++r) {
             int64_t row = t * TR + 16 * r + n;
             if (row >= r_end - r_begin) row = r_end - r_begin - 1;
-            o[r] = (uint32_t)(row * a.xstride) + (uint32_t)(g * lane_bytes<P>());
+            if (a.tiled) {
+                const int64_t R = r_begin + row;
+                o[r] = (uint32_t)(((R >> 4) - (r_begin >> 4)) * 16 * a.xstride + (R & 15) * CB) +
+                       (uint32_t)(g * lane_bytes<P>());
+            } else {
+                o[r] = (uint32_t)(row * a.xstride) + (uint32_t)(g * lane_bytes<P>());
+            }
         }
     };
     auto qfrag_off = [&](int c) {
@@ -584,7 +597,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     // waits assume slot s was requested before slot s + 1)
 #pragma unroll
     for (int s2 = 0; s2 < D; ++s2) {
-        const uint8_t* xb = xbase + (int64_t)(s2 < nchunk ? s2 : nchunk - 1) * CB;
+        const uint8_t* xb = xbase + (int64_t)(s2 < nchunk ? s2 : nchunk - 1) * cstep;
 #pragma unroll
         for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>(xb + cur[r]);
         __builtin_amdgcn_sched_barrier(0);
@@ -650,7 +663,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                 const int cn = c + D;
                 int cc = same ? cn : cn - npad;
                 cc = cc < nchunk ? cc : nchunk - 1;
-                const uint8_t* xb = xbase + (int64_t)cc * CB;
+                const uint8_t* xb = xbase + (int64_t)cc * cstep;
 #pragma unroll
                 for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>(xb + (same ? cur[r] : nxt[r]));
             }
@@ -750,8 +763,8 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     }
     const int qtiles = (a.B + QT - 1) / QT;
     const int64_t rows = a.row_end - a.row_begin;
-    // row chunk: about 8 workgroups per CU over the launch, 2K..32K rows,
-    // whole wave steps
+    // row chunk: about 8 workgroups per CU over the launch, one wave step
+    // (16 * RS rows per wave) to 32K rows, whole wave steps
     constexpr int64_t step = 16 * RS * (NW / QH);
     static const int64_t chunk_env = [] {  // tuning: ANNMI_WIDE_CHUNK_MAX rows
         const char* e = getenv("ANNMI_WIDE_CHUNK_MAX");
@@ -760,7 +773,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     }();
     const int64_t chunk_max = a.chunk_max >= 2048 ? a.chunk_max : chunk_env;
     int64_t chunk = rows * qtiles / ((int64_t)ncu * 8);
-    chunk = chunk < 2048 ? 2048 : (chunk > chunk_max ? chunk_max : chunk);
+    chunk = chunk < step ? step : (chunk > chunk_max ? chunk_max : chunk);
     chunk = (chunk + step - 1) / step * step;
     const int64_t nrc = (rows + chunk - 1) / chunk;
     const int64_t per = (nrc + 7) / 8;
@@ -872,8 +885,13 @@ hipError_t launch_scan_impl(const ScanArgs& a, hipStream_t st) {
     // argmin scans (k-means assignment: few rows, many queries) take the
     // streaming kernel, whose per-lane minima the wide kernel has no
     // registers left for
+    // The wide kernel stages a query tile per row chunk: it takes ranges of
+    // at least 65536 rows, or smaller ones when the batch is large enough
+    // that the streaming kernel's re-reads of every row per 64-query tile
+    // cost more (rows x queries >= 2^24).
     const int min_b = a.wide_min_batch > 0 ? a.wide_min_batch : wide_min_batch(a.kind);
-    if (a.B >= min_b && (a.row_end - a.row_begin) >= 65536 && !a.argmin) {
+    const int64_t nrows = a.row_end - a.row_begin;
+    if (a.B >= min_b && (nrows >= 65536 || nrows * a.B >= ((int64_t)1 << 24)) && !a.argmin) {
         switch (a.kind) {
             case SCAN_F32: return launch_wide<PolF32>(a, st);
             case SCAN_BF16: return launch_wide<PolBF16>(a, st);
