@@ -259,44 +259,11 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) best[w][i] = ~0ull;
 
-    for (int64_t t = t_begin + wave; t < t_end; t += 4) {
+    // Distance, filter and append for the 16-row tile t (acc complete).
+    auto epilogue = [&](int64_t t, typename P::Acc (&acc)[QW]) {
         const int64_t row = a.row_begin + t * 16 + n;
         const bool row_ok = row < a.row_end;
         const int64_t lrow = row_ok ? row : a.row_end - 1;
-        typename P::Acc acc[QW];
-#pragma unroll
-        for (int w = 0; w < QW; ++w) acc[w] = typename P::Acc{};
-
-        constexpr int U = 8;
-        i32x4 buf[U];
-        int c = 0;
-        for (; c + U <= nchunk; c += U) {
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                buf[u] = load_row_frag<P>(X, a.xstride, lrow, (c + u) * P::kChunk + lane_dim, a.tiled);
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-#pragma unroll
-                for (int w = 0; w < QW; ++w) {
-                    const int qoff = (16 * w + n) * ldq;
-                    int eoff = P::kElemBytes == 0 ? ((c + u) * P::kChunk + lane_dim) : ((c + u) * P::kChunk + lane_dim) * P::kElemBytes;
-                    i32x4 qa = *reinterpret_cast<const i32x4*>(smem + qoff + eoff);
-                    P::mma(acc[w], qa, buf[u]);
-                }
-            }
-        }
-        for (; c < nchunk; ++c) {
-            i32x4 b = load_row_frag<P>(X, a.xstride, lrow, c * P::kChunk + lane_dim, a.tiled);
-#pragma unroll
-            for (int w = 0; w < QW; ++w) {
-                const int qoff = (16 * w + n) * ldq;
-                int eoff = P::kElemBytes == 0 ? (c * P::kChunk + lane_dim) : (c * P::kChunk + lane_dim) * P::kElemBytes;
-                i32x4 qa = *reinterpret_cast<const i32x4*>(smem + qoff + eoff);
-                P::mma(acc[w], qa, b);
-            }
-        }
-
-        // epilogue: distance, filter, append
         const float xn = a.xn[lrow];
         if (a.argmin) {
 #pragma unroll
@@ -307,7 +274,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                     unsigned long long kv = ((unsigned long long)f2key(dist) << 32) | (uint32_t)row;
                     if (row_ok && kv < best[w][i]) best[w][i] = kv;
                 }
-            continue;
+            return;
         }
 #pragma unroll
         for (int w = 0; w < QW; ++w) {
@@ -333,6 +300,83 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                     }
                 }
             }
+        }
+    };
+    // acc[w] += query fragments of chunk c (LDS) x row fragment b
+    auto mma_chunk = [&](typename P::Acc (&acc)[QW], int c, const i32x4& b) {
+#pragma unroll
+        for (int w = 0; w < QW; ++w) {
+            const int qoff = (16 * w + n) * ldq;
+            int eoff = P::kElemBytes == 0 ? (c * P::kChunk + lane_dim) : (c * P::kChunk + lane_dim) * P::kElemBytes;
+            i32x4 qa = *reinterpret_cast<const i32x4*>(smem + qoff + eoff);
+            P::mma(acc[w], qa, b);
+        }
+    };
+    const int64_t wt0 = t_begin + wave;  // the wave's tiles: wt0, wt0 + 4, ...
+    // Double-buffered stream (chunk count a multiple of U): the wave's tiles
+    // form one sequence of batches of U chunks, and batch j + 1 (of this tile
+    // or the next) is requested before the MFMAs of batch j, so loads stay in
+    // flight while the wave computes.
+    auto stream = [&](auto uc) {
+        constexpr int U = decltype(uc)::value;
+        const int nb = nchunk / U;
+        const int64_t ntw = wt0 < t_end ? (t_end - wt0 + 3) / 4 : 0;
+        const int64_t npos = ntw * nb;
+        typename P::Acc acc[QW];
+        i32x4 b0[U], b1[U];
+        auto load = [&](i32x4 (&b)[U], int64_t p) {
+            const int64_t t = wt0 + 4 * (p / nb);
+            const int c0 = (int)(p % nb) * U;
+            const int64_t row = a.row_begin + t * 16 + n;
+            const int64_t lrow = row < a.row_end ? row : a.row_end - 1;
+#pragma unroll
+            for (int u = 0; u < U; ++u) b[u] = load_row_frag<P>(X, a.xstride, lrow, (c0 + u) * P::kChunk + lane_dim, a.tiled);
+        };
+        auto step = [&](const i32x4 (&b)[U], int64_t p) {
+            const int cb = (int)(p % nb);
+            if (cb == 0) {
+#pragma unroll
+                for (int w = 0; w < QW; ++w) acc[w] = typename P::Acc{};
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) mma_chunk(acc, cb * U + u, b[u]);
+            if (cb == nb - 1) epilogue(wt0 + 4 * (p / nb), acc);
+        };
+        if (npos > 0) load(b0, 0);
+        for (int64_t p = 0; p < npos; p += 2) {
+            if (p + 1 < npos) load(b1, p + 1);
+            step(b0, p);
+            if (p + 1 < npos) {
+                if (p + 2 < npos) load(b0, p + 2);
+                step(b1, p + 1);
+            }
+        }
+    };
+    // (small query tiles only: with 64 queries the second buffer would cost
+    // the kernel half its waves)
+    if (QW <= 2 && nchunk % 8 == 0) {
+        stream(std::integral_constant<int, 8>{});
+    } else if (QW <= 2 && nchunk % 4 == 0) {
+        stream(std::integral_constant<int, 4>{});
+    } else {
+        for (int64_t t = wt0; t < t_end; t += 4) {
+            const int64_t row = a.row_begin + t * 16 + n;
+            const int64_t lrow = row < a.row_end ? row : a.row_end - 1;
+            typename P::Acc acc[QW];
+#pragma unroll
+            for (int w = 0; w < QW; ++w) acc[w] = typename P::Acc{};
+            constexpr int U = 8;
+            i32x4 buf[U];
+            int c = 0;
+            for (; c + U <= nchunk; c += U) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    buf[u] = load_row_frag<P>(X, a.xstride, lrow, (c + u) * P::kChunk + lane_dim, a.tiled);
+#pragma unroll
+                for (int u = 0; u < U; ++u) mma_chunk(acc, c + u, buf[u]);
+            }
+            for (; c < nchunk; ++c) mma_chunk(acc, c, load_row_frag<P>(X, a.xstride, lrow, c * P::kChunk + lane_dim, a.tiled));
+            epilogue(t, acc);
         }
     }
     if (a.argmin) argmin_flush<QW>(best, a, q0, g, n);
