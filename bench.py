@@ -237,16 +237,19 @@ def measured_traffic(workload: str, mode: str, kernel: str, launches: int):
     summarised in profiles/traffic_<workload>_<mode>.json (FETCH_SIZE and
     WRITE_SIZE over the timed launches of this same command, each scaled by
     the counter's reading for a copy of known size).  None when absent or
-    taken for another kernel or launch count."""
+    taken for another kernel, or for a launch count more than 10% away (the
+    autotuner's timing-based choices can move a few launches between kernel
+    instantiations from run to run)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"traffic_{workload}_{mode}.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kernel or t.get("launches") != launches:
+    if t.get("kernel") != kernel or abs(int(t.get("launches", 0)) - launches) > 0.1 * max(launches, 1):
         return None
     return {"bytes_per_launch": int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]),
+            "launches": int(t["launches"]),
             "source": os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))}
 
 
@@ -263,6 +266,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain fp32 path")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in scan-kernel shapes")
+    ap.add_argument("--fixed-batch", type=int, default=0,
+                    help="every step has this many queries instead of the log-uniform mix (e.g. 1: the "
+                         "one-query HBM-bound case)")
     ap.add_argument("--workload", default="mixed-10m-384", choices=["mixed-10m-384", "ivf-10m-768"])
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only copy this many GiB device to device and exit (HBM counter calibration)")
@@ -290,7 +296,8 @@ def main():
     C, sigma = 4096, 0.3
     log(f"[bench] generating {n} x {d} mixture in HBM")
     base = data.mixture_torch(n, d, C, sigma, 2003, center_seed=1003, normalize=True, device=dev)
-    sizes = batch_sizes(args.warmup + args.steps)
+    sizes = (np.full(args.warmup + args.steps, args.fixed_batch, dtype=int) if args.fixed_batch > 0
+             else batch_sizes(args.warmup + args.steps))
     nq_total = int(sizes.sum())
     Q = data.mixture_torch(nq_total, d, C, sigma, 3003, center_seed=1003, normalize=True, device=dev)
     Qcal = data.mixture_torch(512, d, C, sigma, 3999, center_seed=1003, normalize=True, device=dev)
@@ -389,6 +396,7 @@ def main():
     if tr is not None:
         roof["traffic"] = tr["bytes_per_launch"]
         roof["traffic_source"] = tr["source"]
+        roof["traffic_launches"] = tr["launches"]
     roof["kernel"] = top["kernel"]
     roof["launches"] = int(top["launches"])
     roof["avg_launch_ms"] = round(top["ms"] / max(top["launches"], 1), 4)
@@ -432,7 +440,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"mixed-{n // 1_000_000}m-{d}" if n >= 1_000_000 else f"mixed-{n}-{d}",
                    "n": n, "d": d, "clusters": C, "sigma": sigma, "normalized": True, "k": k,
-                   "batch": "log-uniform [1, 4096]", "queries": nq, "mode": mode, "rerank": R,
+                   "batch": (f"fixed {args.fixed_batch}" if args.fixed_batch > 0 else "log-uniform [1, 4096]"),
+                   "queries": nq, "mode": mode, "rerank": R,
                    "rerank_dtype": "fp32"},
         "recall_at_100": recall,
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
