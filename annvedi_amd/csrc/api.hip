@@ -230,6 +230,10 @@ struct Handle {
     std::vector<StepGraph> graphs;
     uint64_t graph_clock = 0;
     bool tuning = false;  // the autotuner times eager steps (a graph would fix one shape)
+    // Set on a handle that has been cloned and on its clones: handles used
+    // from several threads never capture steps (a capture on one thread and
+    // an allocation, free or legacy-stream operation on another conflict).
+    std::atomic<bool> shared{false};
     // Scan kernel shape per batch size: `fixed` (annmi_set_scan_shape) for
     // every batch, else the autotuned entry of the nearest power of two.
     struct ScanShape {
@@ -593,7 +597,7 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
     Profiler pr{h, ev_used, &scan_ev, &sel_ev, &scan_rows, &scan_names, -1, -1};
     int ev_f0 = -1, ev_f1 = -1;
     Handle::StepGraph* sg = nullptr;
-    if (graphs_enabled() && !h->prof && !h->tuning) {
+    if (graphs_enabled() && !h->prof && !h->tuning && !h->shared.load()) {
         for (auto& g : h->graphs)
             if (g.B == B && g.k == k && g.rsel == rsel && g.ids == ids_out && g.dist == d_out) sg = &g;
         if (!sg) {
@@ -998,7 +1002,8 @@ bool load_mode(Handle* h, const uint8_t* blob) {
         std::vector<uint8_t> t((size_t)(n_alloc * h->xs_stride), 0);
         tile_rows(blob + hd.off[2], t.data(), n, h->xs_stride, tile_cb_of(h->kind));
         HIP_OR(hipMalloc(&h->Xs, t.size()), false);
-        HIP_OR(hipMemcpy(h->Xs, t.data(), t.size(), hipMemcpyHostToDevice), false);
+        HIP_OR(hipMemcpyAsync(h->Xs, t.data(), t.size(), hipMemcpyHostToDevice, st), false);
+        HIP_OR(hipStreamSynchronize(st), false);
         if (!up((void**)&h->xn, 3, n * 4)) return false;
         h->bytes += n_alloc * h->xs_stride + n * 4;
     } else {
@@ -1160,6 +1165,9 @@ Handle* clone_impl(Handle* src) {
     h->list_off = src->list_off;
     h->centroids = src->centroids;
     h->is_clone = true;
+    h->shared = true;
+    src->shared = true;
+    drop_graphs(src);  // the source now runs eagerly too (caller holds its lock)
     if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
         set_err("hipStreamCreate failed");
         delete h;
@@ -1338,7 +1346,10 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
     }
     if (batch_impl(h->cent, h->dQ32, B, nprobe, 0, h->d_probe, h->d_probe_d, true, 0) < 0) return false;
     std::vector<int64_t> probe((size_t)B * nprobe);
-    HIP_OR(hipMemcpy(probe.data(), h->d_probe, probe.size() * 8, hipMemcpyDeviceToHost), false);
+    // on the handle's stream: a legacy-stream copy is refused while another
+    // thread captures a step graph
+    HIP_OR(hipMemcpyAsync(probe.data(), h->d_probe, probe.size() * 8, hipMemcpyDeviceToHost, st), false);
+    HIP_OR(hipStreamSynchronize(st), false);
     // work items: (list chunk x group of <= 16 queries probing that list)
     std::vector<std::vector<uint32_t>> qlist((size_t)h->nlist);
     std::vector<int64_t> scanned((size_t)B, 0);
@@ -1664,7 +1675,8 @@ int64_t annmi_save(void* hv, void* buf, int64_t bytes) {
         if (i == 2 && h->tiled) {  // rows leave the tiled layout: the image is row-major
             const int64_t n_alloc = (h->n + 15) / 16 * 16;
             std::vector<uint8_t> t((size_t)(n_alloc * h->xs_stride));
-            HIP_OR(hipMemcpy(t.data(), h->Xs, t.size(), hipMemcpyDeviceToHost), -1);
+            HIP_OR(hipMemcpyAsync(t.data(), h->Xs, t.size(), hipMemcpyDeviceToHost, h->st), -1);
+            HIP_OR(hipStreamSynchronize(h->st), -1);
             untile_rows(t.data(), out + hd.off[2], h->n, h->xs_stride, tile_cb_of(h->kind));
             continue;
         }
@@ -1675,7 +1687,8 @@ int64_t annmi_save(void* hv, void* buf, int64_t bytes) {
             case 3: case 4: m = h->n * 4; break;
             case 6: m = (int64_t)h->nlist * h->kdim32 * 4; break;
         }
-        HIP_OR(hipMemcpy(out + hd.off[i], src[i], (size_t)m, hipMemcpyDeviceToHost), -1);
+        HIP_OR(hipMemcpyAsync(out + hd.off[i], src[i], (size_t)m, hipMemcpyDeviceToHost, h->st), -1);
+        HIP_OR(hipStreamSynchronize(h->st), -1);
     }
     return need;
 }
@@ -1807,11 +1820,13 @@ int64_t annmi_export(void* hv, int32_t what, void* out, int64_t bytes) {
     if (what == 0 && h->tiled) {  // row-major rows, whatever the device layout
         const int64_t n_alloc = (h->n + 15) / 16 * 16;
         std::vector<uint8_t> t((size_t)(n_alloc * h->xs_stride));
-        HIP_OR(hipMemcpy(t.data(), h->Xs, t.size(), hipMemcpyDeviceToHost), -1);
+        HIP_OR(hipMemcpyAsync(t.data(), h->Xs, t.size(), hipMemcpyDeviceToHost, h->st), -1);
+        HIP_OR(hipStreamSynchronize(h->st), -1);
         untile_rows(t.data(), static_cast<uint8_t*>(out), h->n, h->xs_stride, tile_cb_of(h->kind));
         return need;
     }
-    HIP_OR(hipMemcpy(out, src, (size_t)need, hipMemcpyDeviceToHost), -1);
+    HIP_OR(hipMemcpyAsync(out, src, (size_t)need, hipMemcpyDeviceToHost, h->st), -1);
+    HIP_OR(hipStreamSynchronize(h->st), -1);
     return need;
 }
 

@@ -33,3 +33,25 @@ def test_served_answers_equal_alone_answers(mode):
     print(f"{mode}: {len(batches)} batches, {overl} overlapping pairs, QPS {res['qps']:.0f}, "
           f"p50 {res['p50_ms']:.2f} ms, p99 {res['p99_ms']:.2f} ms")
     assert overl > 0
+
+
+@pytest.mark.parametrize("mode", ["sq8", "ivf"])
+def test_repeated_shapes_on_cloned_handles_across_threads(mode):
+    """Regression: several threads serving on clones of one index with the
+    same request shapes again and again (the shapes a single handle would
+    capture as graphs).  Clones and their source run steps eagerly: no step
+    fails on a capture in one thread meeting copies or frees in another, and
+    every answer equals the answer computed alone."""
+    base, Q = data.workload("exact-1m-384", n=120_000, n_queries=4096)
+    kw = dict(nlist=64, kmeans_iters=4) if mode == "ivf" else {}
+    print(f"[case] repeated shapes on clones {mode}", flush=True)
+    idx = Index(base, mode=mode, rerank=200, **kw)
+    sizes = np.array([1, 4, 8] * 120)
+    with Server(idx, workers=4, max_batch=64, rerank=200, nprobe=8) as srv:
+        res = run_open_loop(srv, Q, sizes, qps=200000, k=100)
+    assert idx.graph_count == 0
+    off = 0
+    for b, (ids, dd) in zip(sizes, res["answers"]):
+        a_ids, a_dd = idx.search(Q[off:off + b], 100, rerank=200, nprobe=8)
+        np.testing.assert_array_equal(ids, a_ids)
+        off += b
