@@ -317,7 +317,7 @@ def main():
         idx = Index(base, mode=mode)
         R = args.rerank
         if R <= 0:
-            for R in (128, 200, 300, 400, 600, 800, 1200, 1600, 2048):
+            for R in (100, 112, 128, 160, 200, 300, 400, 600, 800, 1200, 1600, 2048):
                 ids, _ = idx.search_device(Qcal, k, rerank=R)
                 rc = float(recall_vs_exact(base, Qcal, ids, gt_cal, k).mean())
                 log(f"[bench] calibrate R={R}: recall@{k} = {rc:.4f}")
