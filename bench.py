@@ -220,6 +220,18 @@ def bench_ivf(args):
     print(json.dumps(res), flush=True)
 
 
+def per_launch_roofline(launch_ms, launch_bytes, launch_ops, peak_tops):
+    """(fraction, HBM-bound launch count) of launches against each one's own
+    roofline: sum of max(bytes / HBM peak, ops / MFMA peak) over the measured
+    time.  None without per-launch data."""
+    if not launch_ms or not (len(launch_ms) == len(launch_bytes) == len(launch_ops)):
+        return None
+    t_hbm = [b / (HBM_PEAK_GBS * 1e9) for b in launch_bytes]
+    t_mfma = [f / (peak_tops * 1e12) for f in launch_ops]
+    ideal = sum(max(x, y) for x, y in zip(t_hbm, t_mfma))
+    return round(ideal / (sum(launch_ms) / 1e3), 4), int(sum(1 for x, y in zip(t_hbm, t_mfma) if x >= y))
+
+
 def copy_calibration(gib: float):
     """One device-to-device copy of a known size: what FETCH_SIZE and
     WRITE_SIZE read for it gives their scale on this GPU."""
@@ -413,12 +425,10 @@ def main():
     # are HBM-bound, large ones MFMA-bound), so also report the dominant
     # kernel against its per-launch roofline: sum over launches of
     # max(bytes / HBM peak, ops / MFMA peak), divided by the measured time.
-    lm, lb, lf = top.get("launch_ms", []), top.get("launch_bytes", []), top.get("launch_flops", [])
-    if lm and len(lm) == len(lb) == len(lf):
-        ideal = sum(max(b / (HBM_PEAK_GBS * 1e9), f / (peak_t * 1e12)) for b, f in zip(lb, lf))
-        roof["per_launch_roofline_frac"] = round(ideal / (sum(lm) / 1e3), 4)
-        roof["hbm_bound_launches"] = int(sum(1 for b, f in zip(lb, lf)
-                                             if b / (HBM_PEAK_GBS * 1e9) >= f / (peak_t * 1e12)))
+    plr = per_launch_roofline(top.get("launch_ms", []), top.get("launch_bytes", []), top.get("launch_flops", []),
+                              peak_t)
+    if plr is not None:
+        roof["per_launch_roofline_frac"], roof["hbm_bound_launches"] = plr
 
     cpu = None
     if not args.no_cpu:

@@ -1,0 +1,55 @@
+"""bench.py's contract pieces that need no GPU."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_batch_mix_is_fixed_and_log_uniform():
+    a, b = bench.batch_sizes(23), bench.batch_sizes(23)
+    np.testing.assert_array_equal(a, b)
+    assert a.min() >= 1 and a.max() <= 4096
+    big = bench.batch_sizes(20000, seed=7)
+    # log-uniform: each octave holds about the same share
+    counts = np.histogram(np.log2(big), bins=12, range=(0, 12))[0]
+    assert counts.min() > 0.6 * counts.mean()
+
+
+def test_per_launch_roofline():
+    # one HBM-bound launch at exactly the HBM peak, one MFMA-bound at half
+    ms = [1.0, 2.0]
+    by = [bench.HBM_PEAK_GBS * 1e9 * 1e-3, 1.0]
+    ops = [1.0, 5000e12 * 1e-3]
+    frac, nh = bench.per_launch_roofline(ms, by, ops, 5000.0)
+    assert nh == 1 and abs(frac - 2.0 / 3.0) < 1e-4
+    assert bench.per_launch_roofline([], [], [], 5000.0) is None
+
+
+def test_traffic_profiles_are_consistent_and_matched_by_launch_count():
+    for mode in ("sq8", "exact"):
+        path = os.path.join(ROOT, "profiles", f"traffic_mixed-10m-384_{mode}.json")
+        t = json.load(open(path))
+        cal = t["calibration"]
+        assert abs(t["fetch_bytes_per_launch"] - t["fetch_kib_per_launch"] * 1024 * cal["fetch_scale"]) < 1e-4 * \
+            t["fetch_bytes_per_launch"]
+        assert abs(t["write_bytes_per_launch"] - t["write_kib_per_launch"] * 1024 * cal["write_scale"]) < 1e-4 * \
+            max(t["write_bytes_per_launch"], 1)
+        assert os.path.exists(os.path.join(ROOT, t["summary"]))
+        n = t["launches"]
+        hit = bench.measured_traffic("mixed-10m-384", mode, t["kernel"], n)
+        assert hit["bytes_per_launch"] == int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"])
+        assert bench.measured_traffic("mixed-10m-384", mode, t["kernel"], int(n * 1.5)) is None
+        assert bench.measured_traffic("mixed-10m-384", mode, "another_kernel", n) is None
