@@ -44,17 +44,18 @@ def run_batches(idx, Q, sizes, k, rerank=0):
     """Times every batch (synchronous); returns per-batch seconds and outputs."""
     import torch
 
-    out_ids, times = [], []
+    out_ids, out_d, times = [], [], []
     off = 0
     for b in sizes:
         q = Q[off:off + b]
         off += b
         t0 = time.perf_counter()
-        ids, _ = idx.search_device(q, k, rerank=rerank)
+        ids, dd = idx.search_device(q, k, rerank=rerank)
         times.append(time.perf_counter() - t0)
         out_ids.append(ids)
+        out_d.append(dd)
     torch.cuda.synchronize()
-    return np.array(times), out_ids
+    return np.array(times), out_ids, out_d
 
 
 def recall_vs_exact(base, Q, pred_ids, true_ids, k):
@@ -69,6 +70,17 @@ def recall_vs_exact(base, Q, pred_ids, true_ids, k):
     pd = dist(pred_ids)
     thr = td[:, k - 1:k]
     return (pd[:, :k] <= thr).float().mean(dim=1).cpu().numpy()
+
+
+def dist_error_vs_f64(base, Q, ids, d32):
+    """Largest relative difference between the returned fp32 squared
+    distances and the same rows' squared distances recomputed in fp64."""
+    import torch
+
+    rows = base[ids.reshape(-1)].view(ids.shape[0], ids.shape[1], -1).double()
+    d64 = ((rows - Q[:, None, :].double()) ** 2).sum(dim=2)
+    rel = (d32.double() - d64).abs() / d64.clamp_min(1e-30)
+    return float(rel.max())
 
 
 def cpu_baseline(base_t, Q_t, budget_s=15.0, max_q=200):
@@ -272,8 +284,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="sq8", choices=["bf16", "fp8", "sq8", "sq4", "exact"])
-    ap.add_argument("--n", type=int, default=10_000_000)
-    ap.add_argument("--d", type=int, default=384)
+    ap.add_argument("--n", type=int, default=0, help="rows (0: the workload's)")
+    ap.add_argument("--d", type=int, default=0, help="dims (0: the workload's)")
     ap.add_argument("--k", type=int, default=100)
     ap.add_argument("--rerank", type=int, default=0, help="fixed R (0: calibrate to recall >= 0.99)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain fp32 path")
@@ -282,7 +294,11 @@ def main():
     ap.add_argument("--fixed-batch", type=int, default=0,
                     help="every step has this many queries instead of the log-uniform mix (e.g. 1: the "
                          "one-query HBM-bound case)")
-    ap.add_argument("--workload", default="mixed-10m-384", choices=["mixed-10m-384", "ivf-10m-768"])
+    ap.add_argument("--workload", default="mixed-10m-384",
+                    choices=["mixed-10m-384", "sq-1m-768", "exact-1m-384", "ivf-10m-768"],
+                    help="mixed-10m-384: the headline (10M x 384, batch mix); sq-1m-768: 1M x 768 with per-dim "
+                         "scales (the SQ8/SQ4 configuration); exact-1m-384: 1M x 384 (the plain fp32 drop-in "
+                         "configuration); ivf-10m-768: served IVF")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only copy this many GiB device to device and exit (HBM counter calibration)")
     ap.add_argument("--requests", type=int, default=1500, help="served IVF: requests per rate")
@@ -305,15 +321,17 @@ def main():
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
-    n, d, k = args.n, args.d, args.k
-    C, sigma = 4096, 0.3
+    wn, wd, C, sigma, norm, dscale, zs = data.WORKLOADS[args.workload]
+    cid = {"exact-1m-384": 1, "sq-1m-768": 2, "mixed-10m-384": 3}[args.workload]  # seeds 1000+c, 2000+c, 3000+c
+    n, d, k = args.n or wn, args.d or wd, args.k
+    gen = dict(center_seed=data.SEED_CENTERS + cid, normalize=norm, dim_scale=dscale, zipf_s=zs, device=dev)
     log(f"[bench] generating {n} x {d} mixture in HBM")
-    base = data.mixture_torch(n, d, C, sigma, 2003, center_seed=1003, normalize=True, device=dev)
+    base = data.mixture_torch(n, d, C, sigma, data.SEED_BASE + cid, **gen)
     sizes = (np.full(args.warmup + args.steps, args.fixed_batch, dtype=int) if args.fixed_batch > 0
              else batch_sizes(args.warmup + args.steps))
     nq_total = int(sizes.sum())
-    Q = data.mixture_torch(nq_total, d, C, sigma, 3003, center_seed=1003, normalize=True, device=dev)
-    Qcal = data.mixture_torch(512, d, C, sigma, 3999, center_seed=1003, normalize=True, device=dev)
+    Q = data.mixture_torch(nq_total, d, C, sigma, data.SEED_QUERIES + cid, **gen)
+    Qcal = data.mixture_torch(512, d, C, sigma, 3999, **gen)
     torch.cuda.synchronize()
     log(f"[bench] batches {sizes.tolist()} ({nq_total} queries)")
 
@@ -354,7 +372,7 @@ def main():
     idx.profile(1)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    times, out_ids = run_batches(idx, Qt, tsz, k, R)
+    times, out_ids, out_d = run_batches(idx, Qt, tsz, k, R)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     kstats = idx.profile_kernels()
@@ -370,6 +388,9 @@ def main():
     sel = torch.linspace(0, nq - 1, m, device=dev).long()
     recall = None
     plain = None
+    pred_d = torch.cat(out_d)
+    dist_err = dist_error_vs_f64(base, Qt[sel], pred[sel], pred_d[sel])
+    log(f"[bench] max relative distance error vs fp64 on {m} timed queries: {dist_err:.3g}")
     if mode != "exact":
         gt, _ = exact.search_device(Qt[sel], k)
         recall = float(recall_vs_exact(base, Qt[sel], pred[sel], gt, k).mean())
@@ -377,7 +398,7 @@ def main():
         if not args.no_plain:
             exact.profile(1)
             tp0 = time.perf_counter()
-            ptimes, _ = run_batches(exact, Qt, tsz, k)
+            ptimes, _, _ = run_batches(exact, Qt, tsz, k)
             pwall = time.perf_counter() - tp0
             pprof = exact.profile(0)
             plain = {"qps": round(nq / pwall, 2), "ms_per_step": round(pwall / len(tsz) * 1e3, 3),
@@ -404,7 +425,8 @@ def main():
                 "frac": round(f_mfma, 4)}
     roof["traffic"] = None
     roof["algorithmic_bytes_per_launch"] = int(top["bytes"] / max(top["launches"], 1))
-    tr = measured_traffic(f"mixed-{n // 1_000_000}m-{d}" if n >= 1_000_000 else f"mixed-{n}-{d}", mode, top["kernel"],
+    wname = args.workload if (n, d) == (wn, wd) else f"{args.workload}-n{n}-d{d}"
+    tr = measured_traffic(wname, mode, top["kernel"],
                           int(top["launches"]))
     if tr is not None:
         roof["traffic"] = tr["bytes_per_launch"]
@@ -437,7 +459,9 @@ def main():
         log(f"[bench] cpu baseline {cpu}")
 
     res = {
-        "metric": "QPS at recall@100>=0.99, 10Mx384, 1 MI355X vs plain FP32; p50/p99 ms",
+        "metric": ("QPS at recall@100>=0.99, 10Mx384, 1 MI355X vs plain FP32; p50/p99 ms"
+                   if args.workload == "mixed-10m-384" else
+                   f"QPS at recall@100>=0.99, {args.workload}, 1 MI355X vs plain FP32; p50/p99 ms"),
         "value": round(qps, 2),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -449,12 +473,14 @@ def main():
         "vs_baseline": None,
         "dtype": DTYPE[mode],
         "data": "synthetic",
-        "config": {"workload": f"mixed-{n // 1_000_000}m-{d}" if n >= 1_000_000 else f"mixed-{n}-{d}",
-                   "n": n, "d": d, "clusters": C, "sigma": sigma, "normalized": True, "k": k,
+        "config": {"workload": wname,
+                   "n": n, "d": d, "clusters": C, "sigma": sigma, "normalized": bool(norm),
+                   "dim_scale": dscale, "k": k,
                    "batch": (f"fixed {args.fixed_batch}" if args.fixed_batch > 0 else "log-uniform [1, 4096]"),
                    "queries": nq, "mode": mode, "rerank": R,
                    "rerank_dtype": "fp32"},
         "recall_at_100": recall,
+        "dist_rel_err_max_vs_fp64": dist_err,
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "plain_fp32": plain,
