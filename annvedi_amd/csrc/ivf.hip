@@ -126,10 +126,30 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(IvfScanArgs a) {
             const bool rok = row < item.row_end;
             const int64_t lrow = rok ? row : item.row_end - 1;
             i32x4 acc = (i32x4){0, 0, 0, 0};
-            for (int c = 0; c < nchunk; ++c) {
-                i32x4 b = *reinterpret_cast<const i32x4*>(X + lrow * a.xstride + c * 64 + 16 * g);
+            const uint8_t* xr = X + lrow * a.xstride + 16 * g;
+            const uint8_t* qr = qrow ? qrow + 16 * g : nullptr;
+            // row and query fragments requested U chunks at a time before
+            // their MFMAs (the chunk count is a runtime value, so the
+            // compiler would otherwise wait on every load in turn)
+            constexpr int U = 4;
+            int c = 0;
+            for (; c + U <= nchunk; c += U) {
+                i32x4 b[U], q[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    b[u] = *reinterpret_cast<const i32x4*>(xr + (c + u) * 64);
+                    q[u] = qr ? *reinterpret_cast<const i32x4*>(qr + (c + u) * 64) : (i32x4){0, 0, 0, 0};
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const i32x4 bs = b[u] ^ (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
+                    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(q[u], bs, acc, 0, 0, 0);
+                }
+            }
+            for (; c < nchunk; ++c) {
+                i32x4 b = *reinterpret_cast<const i32x4*>(xr + c * 64);
                 b ^= (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
-                i32x4 q = qrow ? *reinterpret_cast<const i32x4*>(qrow + c * 64 + 16 * g) : (i32x4){0, 0, 0, 0};
+                i32x4 q = qr ? *reinterpret_cast<const i32x4*>(qr + c * 64) : (i32x4){0, 0, 0, 0};
                 acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(q, b, acc, 0, 0, 0);
             }
             const float xn = a.xn[lrow];
