@@ -12,13 +12,18 @@ competitors/gpu_filter/scenarios.yaml, ``refine_r`` as in the ivf_lsh fork)
 is the over-fetch for the quantized modes; IVF takes ``nlist`` (alias
 ``n_clusters``) at fit and ``nprobe`` (alias ``n_probe_clusters``) at query,
 as competitors/ivf_lsh/algorithm.py passes them.  ``ds_size`` truncates the
-base as algorithm.py:19-21 does.
+base as algorithm.py:19-21 does.  ``autotune: true`` times the scan-kernel
+shapes on the fitted index (up to ``max_batch`` queries) before the first
+query; ``index_file`` names a saved index (``Index.save``) to load instead of
+building one, and is written after the build when it does not exist yet.
 
 ``get_n_distances`` counts full-precision distance evaluations: N per query
 in exact mode (src/linear_index.cpp:108), the re-scored rows per query in the
 quantized modes (gpu_filter.cu:171, "only full Euclidean distances count").
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
@@ -55,12 +60,22 @@ class Algorithm:
         nlist = int(index_params.pop("nlist", index_params.pop("n_clusters", 0)) or 0)
         self._nprobe = int(index_params.pop("nprobe", index_params.pop("n_probe_clusters", 0)) or 0)
         kmeans_iters = int(index_params.pop("kmeans_iters", 0) or 0)
+        autotune = bool(index_params.pop("autotune", False))
+        index_file = index_params.pop("index_file", None)
         for unused in ("n_tables", "n_bits"):   # LSH buckets of the ivf_lsh fork: a full-list scan here
             index_params.pop(unused, None)
         if index_params:
             raise ValueError(f"unknown index_params: {sorted(index_params)}")
-        self._index = Index(train, mode=mode, rerank=self._rerank, sq_variant=sq_variant, max_batch=max_batch,
-                            nlist=nlist, nprobe=self._nprobe, kmeans_iters=kmeans_iters)
+        kw = dict(mode=mode, rerank=self._rerank, sq_variant=sq_variant, max_batch=max_batch, nlist=nlist,
+                  nprobe=self._nprobe, kmeans_iters=kmeans_iters)
+        if index_file is not None and os.path.exists(str(index_file)):
+            self._index = Index.load(str(index_file), train, **kw)
+        else:
+            self._index = Index(train, **kw)
+            if index_file is not None:
+                self._index.save(str(index_file))
+        if autotune and mode != "ivf":
+            self._index.autotune(max_batch=max_batch)
 
     def _need_index(self):
         if self._index is None:

@@ -46,3 +46,25 @@ def test_bad_images_are_refused():
     bad[0:8] = b"NOTANIDX"
     with pytest.raises(RuntimeError, match="not an annmi index"):
         Index(base, mode="sq8", saved=bytes(bad))
+
+
+def test_plugin_index_file_and_autotune(tmp_path):
+    """Algorithm.fit(index_file=...) writes the index after the first build and
+    loads it on the next fit; autotune: true tunes at fit; answers agree."""
+    from annvedi_amd import Algorithm
+
+    n, d = 40000, 64
+    print(f"[case] plugin index_file n={n} d={d}", flush=True)
+    base = data.mixture(n, d, 64, 0.3, 91, center_seed=92)
+    Q = data.mixture(20, d, 64, 0.3, 93, center_seed=92)
+    path = str(tmp_path / "plugin.annmi")
+    a = Algorithm()
+    a.fit(base, backend="mi355x", mode="sq8", rerank=300, index_file=path, autotune=True, max_batch=64)
+    import os
+    assert os.path.exists(path)
+    b = Algorithm()
+    b.fit(base, backend="mi355x", mode="sq8", rerank=300, index_file=path)
+    ia, _ = a.batch_query(Q, 100)
+    ib, _ = b.batch_query(Q, 100)
+    np.testing.assert_array_equal(ia, ib)
+    np.testing.assert_array_equal(a.query(Q[0], 100), ib[0])
