@@ -356,6 +356,8 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
     // the kernel half its waves)
     if (QW <= 2 && nchunk % 8 == 0) {
         stream(std::integral_constant<int, 8>{});
+    } else if (QW <= 2 && nchunk % 6 == 0) {  // d = 384 in 64-dim chunks (byte codes), 32-dim (bf16)
+        stream(std::integral_constant<int, 6>{});
     } else if (QW <= 2 && nchunk % 4 == 0) {
         stream(std::integral_constant<int, 4>{});
     } else {
