@@ -671,10 +671,14 @@ bool xn_bounds(Handle* h) {
         // about as tight as the per-row bound and cheaper.  Wider spreads let
         // rows just past a query's threshold through the global bound
         // (measured: fp8 on normalised data, spread ~5%, 11% slower).
+        // Integer dots (SQ8, SQ4) fold each row's norm excess over xn_lo into
+        // the accumulator's start value (mode 3): the cost of the global
+        // bound with the tightness of the per-row one.
         h->prefilter = (hi - lo) <= 0.005f * h->xn_amax ? 2 : 1;
-        if (const char* e = getenv("ANNMI_PREFILTER")) {  // 1 or 2: force a mode (tuning, tests)
+        if (h->kind == SCAN_SQ8 || h->kind == SCAN_SQ4) h->prefilter = 3;
+        if (const char* e = getenv("ANNMI_PREFILTER")) {  // 1, 2 or 3: force a mode (tuning, tests)
             const int v = atoi(e);
-            if (v == 1 || v == 2) h->prefilter = v;
+            if (v >= 1 && v <= 3) h->prefilter = v;
         }
     }
     return true;

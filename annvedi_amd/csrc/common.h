@@ -91,7 +91,8 @@ struct ScanArgs {
     // bounds of xn over all rows: min, max of |xn| (the wide scan's
     // prefilter; the defaults make every pair take the exact test), and the
     // prefilter mode: 1 per-row bound, 2 one bound per query from xn_lo
-    // (chosen when the row norms are nearly equal)
+    // (chosen when the row norms are nearly equal), 3 (integer dots) one
+    // bound per query on dot minus each row's scaled norm excess
     float xn_lo = -__builtin_inff();
     float xn_amax = __builtin_inff();
     int prefilter = 1;
@@ -107,14 +108,16 @@ struct ScanArgs {
     int tiled = 0;
 };
 
-// Wide-kernel tile configurations: queries per workgroup tile (16*QS), rows
-// per wave step (16*RS), waves per workgroup, query fragments prefetched.
+// Wide-kernel tile configurations: queries per workgroup tile, rows per wave
+// step, waves per workgroup, query fragments prefetched, MFMA tile shape.
 enum annmi_wide_cfg : int {
     kWideCfgDefault = 0,   // the built-in choice per element type and dim
     kWideCfgQ8R4W4 = 1,    // 128 queries x 64 rows a wave, 4 waves, prefetched query fragments
     kWideCfgQ8R2W8 = 2,    // 128 x 32, 8 waves (two a SIMD)
     kWideCfgQ4R4W8 = 3,    // 64 x 64, 8 waves
-    kWideCfgCount = 4,
+    kWideCfg32R1 = 4,      // 32 x 32 MFMA tiles (SQ8, BF16): 128 x 32, 8 waves
+    kWideCfg32R2 = 5,      // 32 x 32 MFMA tiles (SQ8, BF16): 128 x 64, 8 waves
+    kWideCfgCount = 6,
 };
 // Whether a wide configuration fits the LDS for this element type and dim.
 bool wide_cfg_fits(int kind, int kdim, int cfg);

@@ -493,6 +493,117 @@ __device__ inline void wide_flush(const ScanArgs& a, const WideQueue& wq, int q0
     __builtin_amdgcn_wave_barrier();
 }
 
+// LDS of a wide-scan workgroup: the query tile (QT rows of ldq bytes), five
+// words per query (threshold, norm, scale, float bound, global-form bound),
+// one queue counter per wave, then the waves' candidate queues.
+struct WideLds {
+    float* thr;
+    float* qn;
+    float* qsc;
+    float* bnd;
+    void* bndg;  // El[QT]
+    uint32_t* cnt;
+};
+
+// Stage a workgroup's query tile and per-query bounds (every thread takes
+// part; ends with a barrier).  Queries past B are zero rows with threshold
+// -inf, which no pair passes.
+template <class El>
+__device__ inline WideLds wide_stage(const ScanArgs& a, uint8_t* smem, int QT, int ldq, int qbytes, int q0) {
+    WideLds L;
+    L.thr = reinterpret_cast<float*>(smem + QT * ldq);
+    L.qn = L.thr + QT;
+    L.qsc = L.qn + QT;
+    L.bnd = L.qsc + QT;
+    El* bndg_s = reinterpret_cast<El*>(L.bnd + QT);
+    L.bndg = bndg_s;
+    L.cnt = reinterpret_cast<uint32_t*>(bndg_s + QT);
+    const int nthr = (int)blockDim.x;
+    // the query tile: eight 16-byte loads per lane in flight
+    {
+        const int per_row = qbytes / 16;
+        const int total = QT * per_row;
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < total; i0 += nthr * U) {
+            i32x4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = i0 + u * nthr + (int)threadIdx.x;
+                const int r = idx / per_row;
+                v[u] = (i32x4){0, 0, 0, 0};
+                if (idx < total && q0 + r < a.B)
+                    v[u] = reinterpret_cast<const i32x4*>(static_cast<const uint8_t*>(a.Q) +
+                                                          (int64_t)(q0 + r) * qbytes)[idx - r * per_row];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = i0 + u * nthr + (int)threadIdx.x;
+                const int r = idx / per_row;
+                if (idx < total) *reinterpret_cast<i32x4*>(smem + r * ldq + (idx - r * per_row) * 16) = v[u];
+            }
+        }
+    }
+    for (int m = threadIdx.x; m < QT; m += nthr) {
+        int qi = q0 + m;
+        bool ok = qi < a.B;
+        const float th = ok ? a.thr[qi] : -__builtin_inff();
+        const float qv = ok ? a.qn[qi] : 0.f;
+        const float qs = (ok && a.qscale) ? a.qscale[qi] : 1.f;
+        L.thr[m] = th;
+        L.qn[m] = qv;
+        L.qsc[m] = qs;
+        // Prefilter bound: a pair passes dist = (xn + qv) - 2*qs*dot <= th
+        // only if 2*qs*dot - xn >= qv - th - (rounding), so that value
+        // reaching bnd[m] is necessary.  The 2^-14 relative margin is
+        // ~1000x the fp32 rounding of the test; the exact test decides.
+        // th = -inf (padding) gives NaN: never passes; th = +inf: -inf.
+        const float M = a.xn_amax + fabsf(qv) + fabsf(th);
+        L.bnd[m] = 0.5f * ((qv - th) - M * 0x1p-14f);  // halved: the test scales by 1/2
+        // global form (prefilter 2): xn >= xn_lo for every row, so
+        // dot >= ((xn_lo + qv) - th - margin) / (2 qs) is necessary
+        const float sc = 2.0f * qs;
+        const float num = ((a.xn_lo + qv) - th) - M * 0x1p-14f;
+        const float bg = sc > 0.f ? num / sc : (num <= 0.f ? -__builtin_inff() : __builtin_inff());
+        if constexpr (std::is_same<El, float>::value) {
+            bndg_s[m] = bg;
+        } else {  // integer dot: the smallest integer >= bg; NaN (padding) never passes
+            bndg_s[m] = bg != bg ? 0x7fffffff
+                                 : (bg <= -2147483648.0f ? (int)0x80000000
+                                                         : (bg >= 2147483647.0f ? 0x7fffffff : (int)ceilf(bg)));
+        }
+    }
+    if ((int)threadIdx.x < nthr / 64) L.cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    return L;
+}
+
+// Prefilter 3 (integer dots): the accumulators of row r start at -D_r,
+// D_r = floor((xn_r - xn_lo) * kap) with kap = 1 / (2 * max qs over the
+// tile's queries) <= 1 / (2 qs) for each of them.  A pass needs
+// dot >= ((xn_lo + qv) - th - margin) / (2 qs) + (xn_r - xn_lo) / (2 qs),
+// so dot - D_r >= the global form's integer bound is necessary, and one
+// integer compare per (query, lane) is as tight as the per-row bound when
+// the tile's query scales are equal.  The exact test adds D_r back, so it
+// sees the same integer dot.  Returns 0 (no offsets) in the other modes.
+__device__ inline float wide_kap(const ScanArgs& a, const WideLds& L, int QT, int lane) {
+    if (a.prefilter != 3) return 0.f;
+    float mq = 0.f;
+    for (int m = lane; m < QT; m += 64)
+        if (L.thr[m] != -__builtin_inff()) mq = fmaxf(mq, L.qsc[m]);
+#pragma unroll
+    for (int o = 32; o; o >>= 1) mq = fmaxf(mq, __shfl_xor(mq, o));
+    // the factor keeps the rounding of kap and of the product below the
+    // exact quotient
+    return (mq > 0.f && mq < __builtin_inff()) ? (0.5f / mq) * (1.0f - 0x1p-20f) : 0.f;
+}
+
+// Row offset D = floor((xn - xn_lo) * kap) of prefilter 3, clamped to
+// [0, 2^24] (a smaller offset keeps the bound necessary); 0 when kap is 0.
+__device__ inline int wide_row_offset(const ScanArgs& a, float xn, float kap) {
+    const float v = (xn - a.xn_lo) * kap;  // NaN (kap 0, xn_lo -inf) gives 0
+    return v > 0.f ? (v < 16777216.f ? (int)v : 16777216) : 0;
+}
+
 template <class P, int QS, int RS, int kWideWaves, int QH, bool kQPre>
 __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, int qtiles, int64_t chunk_rows,
                                                                     int64_t nrc, int qcap) {
@@ -521,74 +632,20 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     const int q0 = qtile * QT;
     const int qbytes = P::kElemBytes == 0 ? a.kdim : a.kdim * P::kElemBytes;
     const int ldq = qrow_bytes<P>(a.kdim);
-    float* thr_s = reinterpret_cast<float*>(smem + QT * ldq);
-    float* qn_s = thr_s + QT;
-    float* qsc_s = qn_s + QT;
-    float* bnd_s = qsc_s + QT;
     typedef typename P::El El;
-    El* bndg_s = reinterpret_cast<El*>(bnd_s + QT);
-    uint32_t* cnt_s = reinterpret_cast<uint32_t*>(bndg_s + QT);  // [kWideWaves]
+    const WideLds L = wide_stage<El>(a, smem, QT, ldq, qbytes, q0);
+    const float* thr_s = L.thr;
+    const float* qn_s = L.qn;
+    const float* qsc_s = L.qsc;
+    const float* bnd_s = L.bnd;
+    const El* bndg_s = static_cast<const El*>(L.bndg);
     WideQueue wq;
-    wq.row = cnt_s + kWideWaves + (int64_t)wave * 2 * qcap;
+    wq.row = L.cnt + kWideWaves + (int64_t)wave * 2 * qcap;
     wq.key = reinterpret_cast<float*>(wq.row + qcap);
-    wq.count = cnt_s + wave;
+    wq.count = L.cnt + wave;
     wq.cap = qcap;
-
-    // stage the query tile: eight 16-byte loads per lane in flight
-    {
-        const int per_row = qbytes / 16;
-        const int total = QT * per_row;
-        constexpr int U = 8;
-        for (int i0 = 0; i0 < total; i0 += 64 * kWideWaves * U) {
-            i32x4 v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int idx = i0 + u * 64 * kWideWaves + (int)threadIdx.x;
-                const int r = idx / per_row;
-                v[u] = (i32x4){0, 0, 0, 0};
-                if (idx < total && q0 + r < a.B)
-                    v[u] = reinterpret_cast<const i32x4*>(static_cast<const uint8_t*>(a.Q) +
-                                                          (int64_t)(q0 + r) * qbytes)[idx - r * per_row];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int idx = i0 + u * 64 * kWideWaves + (int)threadIdx.x;
-                const int r = idx / per_row;
-                if (idx < total) *reinterpret_cast<i32x4*>(smem + r * ldq + (idx - r * per_row) * 16) = v[u];
-            }
-        }
-    }
-    for (int m = threadIdx.x; m < QT; m += 64 * kWideWaves) {
-        int qi = q0 + m;
-        bool ok = qi < a.B;
-        const float th = ok ? a.thr[qi] : -__builtin_inff();
-        const float qv = ok ? a.qn[qi] : 0.f;
-        const float qs = (ok && a.qscale) ? a.qscale[qi] : 1.f;
-        thr_s[m] = th;
-        qn_s[m] = qv;
-        qsc_s[m] = qs;
-        // Prefilter bound: a pair passes dist = (xn + qv) - 2*qs*dot <= th
-        // only if 2*qs*dot - xn >= qv - th - (rounding), so that value
-        // reaching bnd_s[m] is necessary.  The 2^-14 relative margin is
-        // ~1000x the fp32 rounding of the test; the exact test decides.
-        // th = -inf (padding) gives NaN: never passes; th = +inf: -inf.
-        const float M = a.xn_amax + fabsf(qv) + fabsf(th);
-        bnd_s[m] = 0.5f * ((qv - th) - M * 0x1p-14f);  // halved: the test scales by 1/2
-        // global form (prefilter 2): xn >= xn_lo for every row, so
-        // dot >= ((xn_lo + qv) - th - margin) / (2 qs) is necessary
-        const float sc = 2.0f * qs;
-        const float num = ((a.xn_lo + qv) - th) - M * 0x1p-14f;
-        const float bg = sc > 0.f ? num / sc : (num <= 0.f ? -__builtin_inff() : __builtin_inff());
-        if constexpr (std::is_same<El, float>::value) {
-            bndg_s[m] = bg;
-        } else {  // integer dot: the smallest integer >= bg; NaN (padding) never passes
-            bndg_s[m] = bg != bg ? 0x7fffffff
-                                 : (bg <= -2147483648.0f ? (int)0x80000000
-                                                         : (bg >= 2147483647.0f ? 0x7fffffff : (int)ceilf(bg)));
-        }
-    }
-    if (threadIdx.x < kWideWaves) cnt_s[threadIdx.x] = 0u;
-    __syncthreads();
+    constexpr bool kIntDot = std::is_same<El, int>::value;
+    const float kap = kIntDot ? wide_kap(a, L, QT, lane) : 0.f;
 
     const uint8_t* X = static_cast<const uint8_t*>(a.X);
     const int nchunk = a.kdim / P::kChunk;
@@ -655,19 +712,38 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
             qa[0][q] = *reinterpret_cast<const i32x4*>(smem + (qbase + 16 * q + n) * ldq + qfrag_off(0));
     }
 
+    // row norms, loaded a tile ahead: the first MFMAs of a tile take them
+    // (prefilter 3) without waiting for that tile's chunk prefetches
+    float xnn[RS];
+    auto load_xn = [&](int64_t t) {
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            const int64_t row = r_begin + t * TR + 16 * r + n;
+            xnn[r] = a.xn[row < r_end ? row : r_end - 1];
+        }
+    };
+    load_xn(stream);
+
     for (int64_t t = stream; t < ntiles; t += NS) {
         const int64_t r0 = r_begin + t * TR;
         tile_offs(t, cur);
         tile_offs(t + NS < ntiles ? t + NS : t, nxt);
-        // row norms, requested before this tile's chunk prefetches so that
-        // the epilogue's wait for them leaves those in flight
         bool rok[RS];
         float xn[RS];
 #pragma unroll
         for (int r = 0; r < RS; ++r) {
-            const int64_t row = r0 + 16 * r + n;
-            rok[r] = row < r_end;
-            xn[r] = a.xn[rok[r] ? row : r_end - 1];
+            rok[r] = r0 + 16 * r + n < r_end;
+            xn[r] = xnn[r];
+        }
+        load_xn(t + NS < ntiles ? t + NS : t);
+        int dofs[RS];
+        typename P::Acc cinit[RS];
+        if constexpr (kIntDot) {
+#pragma unroll
+            for (int r = 0; r < RS; ++r) {
+                dofs[r] = wide_row_offset(a, xn[r], kap);
+                cinit[r] = (i32x4){-dofs[r], -dofs[r], -dofs[r], -dofs[r]};
+            }
         }
         typename P::Acc acc[QS][RS];
         // One ring turn (D chunks).  The first turn of a tile starts the
@@ -700,9 +776,14 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                     for (int q = 0; q < QS; ++q)
 #pragma unroll
                         for (int r = 0; r < RS; ++r) {
-                            if (kFirst && s2 == 0)
-                                P::mma0(acc[q][r], qa[s2 & 1][q], bb[r]);
-                            else
+                            if (kFirst && s2 == 0) {
+                                if constexpr (kIntDot) {
+                                    acc[q][r] = cinit[r];
+                                    P::mma(acc[q][r], qa[s2 & 1][q], bb[r]);
+                                } else {
+                                    P::mma0(acc[q][r], qa[s2 & 1][q], bb[r]);
+                                }
+                            } else
                                 P::mma(acc[q][r], qa[s2 & 1][q], bb[r]);
                         }
                 }
@@ -726,17 +807,30 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
         // equal, max(dot) against one bound per query instead.  Only lanes
         // and query groups that may pass evaluate the exact test and append,
         // and a tile where no lane of the wave may pass skips that part.
+        // Prefilters 2 and 3 compare max(dot) (max(dot - D_r) under 3) with
+        // one bound per query.
         uint32_t pend = 0;  // bit q: query group q may pass in this lane
-        if (a.prefilter == 2) {
+        if (a.prefilter >= 2) {
 #pragma unroll
             for (int q = 0; q < QS; ++q) {
+                const int m0 = qbase + 16 * q + 4 * g;
+                El bg4[4];
+                if constexpr (kIntDot) {
+                    const i32x4 v = *reinterpret_cast<const i32x4*>(bndg_s + m0);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) bg4[i] = v[i];
+                } else {
+                    const f32x4 v = *reinterpret_cast<const f32x4*>(bndg_s + m0);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) bg4[i] = v[i];
+                }
                 bool maybe = false;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     El mx = acc[q][0][i];
 #pragma unroll
                     for (int r = 1; r < RS; ++r) mx = acc[q][r][i] > mx ? acc[q][r][i] : mx;
-                    maybe |= mx >= bndg_s[qbase + 16 * q + 4 * g + i];
+                    maybe |= mx >= bg4[i];
                 }
                 pend |= maybe ? 1u << q : 0u;
             }
@@ -771,7 +865,12 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                     const float th = thr_s[m], qv = qn_s[m], sc = 2.0f * qsc_s[m];
 #pragma unroll
                     for (int r = 0; r < RS; ++r) {
-                        const float dist = xn[r] + qv - sc * (float)acc[q][r][i];
+                        float dot;
+                        if constexpr (kIntDot)
+                            dot = (float)(acc[q][r][i] + dofs[r]);
+                        else
+                            dot = (float)acc[q][r][i];
+                        const float dist = xn[r] + qv - sc * dot;
                         if (rok[r] && dist <= th) {
                             const uint32_t roff = (uint32_t)(r0 - r_begin + 16 * r + n);
                             wide_append(a, wq, q0 + m, m, roff, (uint32_t)(r_begin + roff), dist);
@@ -837,6 +936,353 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Wide scan on 32 x 32 MFMA tiles (SQ8: v_mfma_i32_32x32x32_i8, BF16:
+// v_mfma_f32_32x32x16_bf16), an autotuner candidate.  One 32 x 32 MFMA does
+// twice the work of a 16 x 16 one from the same operand bytes in the same
+// issue time (measured on MI355X with register operands only: int8 4.9
+// POP/s against 2.8 for 16x16x64; bf16 16x16x32 1.44 PFLOP/s), so the
+// 16 x 16 kernel above cannot pass ~57% of the int8 and bf16 peaks.  This
+// kernel is bound before that by its LDS and load stream (SQ8 10M x 384 at
+// 4096 queries: 1.8 POP/s with everything, 2.7 without the epilogue and the
+// row loads), so it is not the default.  Same workgroup plan, query
+// staging, prefilters and queues; per wave step a 32 * RS row tile against
+// the 32 * QS query tile.
+//
+// Operands: lane l holds index l % 32 (query of A, row of B) over bytes
+// [16 * (l / 32), +16) of a 32-byte K step (the same K order for A and B,
+// which is all the dot needs).  Accumulator element e of lane l is row
+// (query) 8 * (e / 4) + 4 * (l / 32) + e % 4, column (row) l % 32.
+//
+// Rows in the tiled layout (16-row tiles of 64-byte chunks): K step k of
+// row R lies at tile(R) + (k / 2) * 1024 + (R % 16) * 64 + (k % 2) * 32.
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct W32SQ8 {
+    static constexpr int kElemBytes = 1;
+    typedef int El;
+    typedef i32x16 Acc;
+    __device__ static inline void mma(Acc& acc, const i32x4& a, const i32x4& b) {
+        // codes are uint8; the MFMA reads int8: c - 128 (the query side adds
+        // 128 * sum(w) back)
+        const i32x4 bs = b ^ (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bs, acc, 0, 0, 0);
+    }
+    static const char* name() { return "sq8"; }
+};
+struct W32BF16 {
+    static constexpr int kElemBytes = 2;
+    typedef float El;
+    typedef f32x16 Acc;
+    __device__ static inline void mma(Acc& acc, const i32x4& a, const i32x4& b) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                      0, 0, 0);
+    }
+    static const char* name() { return "bf16"; }
+};
+
+template <class W, int QS, int RS, int kWaves, int D, bool kQPre>
+__global__ __launch_bounds__(64 * kWaves) void scan_wide32_kernel(ScanArgs a, int qtiles, int64_t chunk_rows,
+                                                                   int64_t nrc, int qcap) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    typedef typename W::El El;
+    typedef typename W::Acc Acc;
+    constexpr bool kIntDot = std::is_same<El, int>::value;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int h = lane >> 5;  // K half of the operands; row quad of the outputs
+    const int j = lane & 31;  // operand index; output column
+    // work item of this workgroup: XCD x owns row chunks [x*per, (x+1)*per)
+    const int xcd = blockIdx.x & 7;
+    const int64_t s = blockIdx.x >> 3;
+    const int64_t per = (nrc + 7) / 8;
+    if (s / qtiles >= per) return;
+    const int64_t rc = xcd * per + s / qtiles;
+    if (rc >= nrc) return;
+    const int qtile = (int)(s % qtiles);
+    const int64_t r_begin = a.row_begin + rc * chunk_rows;
+    const int64_t r_end = r_begin + chunk_rows < a.row_end ? r_begin + chunk_rows : a.row_end;
+
+    constexpr int QT = 32 * QS;
+    static_assert(QT <= 128, "queue entries hold the query slot in 7 bits");
+    const int q0 = qtile * QT;
+    const int qbytes = a.kdim * W::kElemBytes;
+    const int ldq = qbytes + 16;
+    const WideLds L = wide_stage<El>(a, smem, QT, ldq, qbytes, q0);
+    const El* bndg_s = static_cast<const El*>(L.bndg);
+    WideQueue wq;
+    wq.row = L.cnt + kWaves + (int64_t)wave * 2 * qcap;
+    wq.key = reinterpret_cast<float*>(wq.row + qcap);
+    wq.count = L.cnt + wave;
+    wq.cap = qcap;
+    const float kap = kIntDot ? wide_kap(a, L, QT, lane) : 0.f;
+
+    const uint8_t* X = static_cast<const uint8_t*>(a.X);
+    // K steps of a row; D (K steps of row fragments in flight per wave)
+    // divides it, so every ring slot of a turn multiplies
+    const int nks = qbytes / 32;
+    constexpr int TR = 32 * RS;  // rows per wave step
+    const int npad = nks;
+    const int64_t ntiles = (r_end - r_begin + TR - 1) / TR;
+    if (wave >= ntiles) return;
+    const bool tiled = a.tiled != 0;
+    const uint8_t* xbase = tiled ? X + (r_begin >> 4) * 16 * a.xstride : X + r_begin * a.xstride;
+    // uniform byte offset of K step k from a row's offset
+    auto koff = [&](int k) -> int64_t { return tiled ? (int64_t)(k >> 1) * 1024 + (k & 1) * 32 : (int64_t)k * 32; };
+    // per-lane row offsets of wave step t (rows past the range repeat the last)
+    auto tile_offs = [&](int64_t t, uint32_t (&o)[RS]) {
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            int64_t row = t * TR + 32 * r + j;
+            if (row >= r_end - r_begin) row = r_end - r_begin - 1;
+            if (tiled) {
+                const int64_t R = r_begin + row;
+                o[r] = (uint32_t)(((R >> 4) - (r_begin >> 4)) * 16 * a.xstride + (R & 15) * 64) + (uint32_t)(16 * h);
+            } else {
+                o[r] = (uint32_t)(row * a.xstride) + (uint32_t)(16 * h);
+            }
+        }
+    };
+    const uint8_t* qlane = smem + j * ldq + 16 * h;  // this lane's A bytes of query tile 0, K step 0
+
+    i32x4 ring[D][RS];
+    uint32_t cur[RS], nxt[RS];
+    tile_offs(wave, cur);
+    // issued in slot order: the loop's waits assume slot s was requested
+    // before slot s + 1
+#pragma unroll
+    for (int s2 = 0; s2 < D; ++s2) {
+        const uint8_t* xb = xbase + koff(s2);
+#pragma unroll
+        for (int r = 0; r < RS; ++r) ring[s2][r] = *reinterpret_cast<const i32x4*>(xb + cur[r]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // row norms a wave step ahead
+    float xnn[RS];
+    auto load_xn = [&](int64_t t) {
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            const int64_t row = r_begin + t * TR + 32 * r + j;
+            xnn[r] = a.xn[row < r_end ? row : r_end - 1];
+        }
+    };
+    load_xn(wave);
+    // kQPre: query fragments alternate between two register sets, K step c
+    // multiplying qa[c & 1] while the next step's are read (D and the K step
+    // count are even, so the parity carries across turns and wave steps)
+    i32x4 qa[2][QS];
+    if constexpr (kQPre) {
+#pragma unroll
+        for (int q = 0; q < QS; ++q) qa[0][q] = *reinterpret_cast<const i32x4*>(qlane + 32 * q * ldq);
+    }
+
+    for (int64_t t = wave; t < ntiles; t += kWaves) {
+        const int64_t r0 = r_begin + t * TR;
+        tile_offs(t, cur);
+        tile_offs(t + kWaves < ntiles ? t + kWaves : t, nxt);
+        bool rok[RS];
+        float xn[RS];
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            rok[r] = r0 + 32 * r + j < r_end;
+            xn[r] = xnn[r];
+        }
+        load_xn(t + kWaves < ntiles ? t + kWaves : t);
+        int dofs[RS];
+        Acc cinit[RS];
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            if constexpr (kIntDot) {
+                dofs[r] = wide_row_offset(a, xn[r], kap);
+#pragma unroll
+                for (int e = 0; e < 16; ++e) cinit[r][e] = -dofs[r];
+            } else {
+                dofs[r] = 0;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) cinit[r][e] = 0.f;
+            }
+        }
+        Acc acc[QS][RS];
+        // One ring turn (D K steps); the first turn of a wave step starts
+        // the accumulators from cinit.
+        auto turn = [&](int c0, auto first) {
+            constexpr bool kFirst = decltype(first)::value;
+            const bool same = c0 + D < npad;  // all slots prefetch from this wave step
+#pragma unroll
+            for (int s2 = 0; s2 < D; ++s2) {
+                const int c = c0 + s2;
+                if constexpr (kQPre) {
+                    const int c1 = c + 1 < nks ? c + 1 : 0;
+#pragma unroll
+                    for (int q = 0; q < QS; ++q)
+                        qa[(s2 + 1) & 1][q] = *reinterpret_cast<const i32x4*>(qlane + 32 * q * ldq + 32 * c1);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < QS; ++q)
+                        qa[s2 & 1][q] = *reinterpret_cast<const i32x4*>(qlane + 32 * q * ldq + 32 * c);
+                }
+#pragma unroll
+                for (int q = 0; q < QS; ++q)
+#pragma unroll
+                    for (int r = 0; r < RS; ++r) {
+                        if (kFirst && s2 == 0) acc[q][r] = cinit[r];
+                        W::mma(acc[q][r], qa[s2 & 1][q], ring[s2][r]);
+                    }
+                const int cn = c + D;
+                const int cc = same ? cn : cn - npad;
+                const uint8_t* xb = xbase + koff(cc);
+#pragma unroll
+                for (int r = 0; r < RS; ++r) ring[s2][r] = *reinterpret_cast<const i32x4*>(xb + (same ? cur[r] : nxt[r]));
+            }
+        };
+        turn(0, std::true_type{});
+        for (int c0 = D; c0 < npad; c0 += D) turn(c0, std::false_type{});
+
+        // epilogue: prefilter per (query group q, lane), as in the 16 x 16
+        // kernel (modes 2 and 3: max over the lane's rows of dot (- D_r)
+        // against one bound per query; mode 1: qs * max(dot) - min(xn) / 2
+        // against the per-query bound), then the exact test where it may pass
+        uint32_t pend = 0;
+        if (a.prefilter >= 2) {
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                bool maybe = false;
+#pragma unroll
+                for (int e4 = 0; e4 < 4; ++e4) {
+                    const int m0 = 32 * q + 8 * e4 + 4 * h;
+                    El bg4[4];
+                    if constexpr (kIntDot) {
+                        const i32x4 v = *reinterpret_cast<const i32x4*>(bndg_s + m0);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) bg4[i] = v[i];
+                    } else {
+                        const f32x4 v = *reinterpret_cast<const f32x4*>(bndg_s + m0);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) bg4[i] = v[i];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        El mx = acc[q][0][4 * e4 + i];
+#pragma unroll
+                        for (int r = 1; r < RS; ++r) mx = acc[q][r][4 * e4 + i] > mx ? acc[q][r][4 * e4 + i] : mx;
+                        maybe |= mx >= bg4[i];
+                    }
+                }
+                pend |= maybe ? 1u << q : 0u;
+            }
+        } else {
+            float hx = xn[0];
+#pragma unroll
+            for (int r = 1; r < RS; ++r) hx = fminf(hx, xn[r]);
+            hx *= 0.5f;
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                bool maybe = false;
+#pragma unroll
+                for (int e4 = 0; e4 < 4; ++e4) {
+                    const int m0 = 32 * q + 8 * e4 + 4 * h;
+                    const f32x4 qs4 = *reinterpret_cast<const f32x4*>(L.qsc + m0);
+                    const f32x4 bn4 = *reinterpret_cast<const f32x4*>(L.bnd + m0);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        El mx = acc[q][0][4 * e4 + i];
+#pragma unroll
+                        for (int r = 1; r < RS; ++r) mx = acc[q][r][4 * e4 + i] > mx ? acc[q][r][4 * e4 + i] : mx;
+                        maybe |= fmaf(qs4[i], (float)mx, -hx) >= bn4[i];
+                    }
+                }
+                pend |= maybe ? 1u << q : 0u;
+            }
+        }
+        if (__ballot(pend != 0u) != 0) {
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                if (!((pend >> q) & 1u)) continue;
+#pragma unroll
+                for (int e4 = 0; e4 < 4; ++e4) {
+                    const int m0 = 32 * q + 8 * e4 + 4 * h;
+                    const f32x4 th4 = *reinterpret_cast<const f32x4*>(L.thr + m0);
+                    const f32x4 qv4 = *reinterpret_cast<const f32x4*>(L.qn + m0);
+                    const f32x4 qs4 = *reinterpret_cast<const f32x4*>(L.qsc + m0);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float sc = 2.0f * qs4[i];
+#pragma unroll
+                        for (int r = 0; r < RS; ++r) {
+                            float dot;
+                            if constexpr (kIntDot)
+                                dot = (float)(acc[q][r][4 * e4 + i] + dofs[r]);
+                            else
+                                dot = (float)acc[q][r][4 * e4 + i];
+                            const float dist = xn[r] + qv4[i] - sc * dot;
+                            if (rok[r] && dist <= th4[i]) {
+                                const uint32_t roff = (uint32_t)(r0 - r_begin + 32 * r + j);
+                                wide_append(a, wq, q0 + m0 + i, m0 + i, roff, (uint32_t)(r_begin + roff), dist);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // keep room for a wave step's typical appends
+        if (__builtin_amdgcn_readfirstlane(*wq.count) + 512u > (uint32_t)qcap) wide_flush(a, wq, q0, r_begin, lane);
+    }
+    wide_flush(a, wq, q0, r_begin, lane);
+}
+
+template <class W, int QS, int RS, int NW, int D, bool QP>
+hipError_t launch_wide32_t(const ScanArgs& a, hipStream_t st) {
+    static char name[80] = "";
+    if (!name[0])
+        snprintf(name, sizeof(name), "scan_wide32_kernel<%s,QS=%d,RS=%d,NW=%d,D=%d%s>", W::name(), QS, RS, NW, D,
+                 QP ? ",QP" : "");
+    t_variant = name;
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 256;
+    }
+    constexpr int QT = 32 * QS;
+    const int qbytes = a.kdim * W::kElemBytes;
+    if (qbytes % (32 * D) != 0) return hipErrorInvalidValue;  // whole ring turns of 32-byte K steps
+    const size_t lds = (size_t)QT * (qbytes + 16) + 5 * QT * sizeof(float) + NW * sizeof(uint32_t);
+    if (lds + 8 * 1024 > 160 * 1024) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide32_kernel<W, QS, RS, NW, D, QP>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    const int qtiles = (a.B + QT - 1) / QT;
+    const int64_t rows = a.row_end - a.row_begin;
+    // row chunk as for the 16 x 16 kernel: about 8 workgroups per CU over
+    // the launch, whole wave steps, at most 32K rows (or the tuned maximum)
+    constexpr int64_t step = 32 * RS * NW;
+    static const int64_t chunk_env = [] {
+        const char* e = getenv("ANNMI_WIDE_CHUNK_MAX");
+        const long v = e ? atol(e) : 0;
+        return v >= 2048 ? (int64_t)v : (int64_t)32768;
+    }();
+    const int64_t chunk_max = a.chunk_max >= 2048 ? a.chunk_max : chunk_env;
+    int64_t chunk = rows * qtiles / ((int64_t)ncu * 8);
+    chunk = chunk < step ? step : (chunk > chunk_max ? chunk_max : chunk);
+    chunk = (chunk + step - 1) / step * step;
+    const int64_t nrc = (rows + chunk - 1) / chunk;
+    const int64_t per = (nrc + 7) / 8;
+    int qcap = (int)((160 * 1024 - lds) / (NW * 8)) / 64 * 64;
+    if (qcap > 4096) qcap = 4096;
+    if (const char* e = getenv("ANNMI_WIDE_QCAP")) {  // lower cap: tuning and tests
+        const int v = atoi(e);
+        if (v >= 0 && v < qcap) qcap = v / 64 * 64;
+    }
+    if (qcap < 1024) qcap = 0;
+    hipLaunchKernelGGL((scan_wide32_kernel<W, QS, RS, NW, D, QP>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
+                       lds + (size_t)NW * qcap * 8, st, a, qtiles, chunk, nrc, qcap);
+    return hipGetLastError();
+}
+
 // LDS of a wide configuration's query tile: 16*QS queries of qrow bytes plus
 // five per-query floats (the queues take what is left of the 160 KB).
 template <class P>
@@ -849,8 +1295,40 @@ bool wide_fits(int qs, int kdim) {
 // default picks per element type: 128 x 64 tiles on 4 waves for fp32, 128 x
 // 32 tiles on 8 waves for bf16 and the byte-sized codes, smaller query tiles
 // where 128 queries do not fit the LDS.
+// the 32 x 32 MFMA form of a scan type (void: none)
+template <class P>
+struct W32Of {
+    typedef void T;
+};
+template <>
+struct W32Of<PolSQ8> {
+    typedef W32SQ8 T;
+};
+template <>
+struct W32Of<PolBF16> {
+    typedef W32BF16 T;
+};
+
 template <class P>
 hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
+    typedef typename W32Of<P>::T W;
+    if constexpr (!std::is_void<W>::value) {
+        if (a.wide_cfg == kWideCfg32R1 || a.wide_cfg == kWideCfg32R2) {
+            // ring depth: the deepest of 8, 6, 4, 2 K steps dividing the row's
+            // (rows are whole 64-byte chunks: an even count); two row tiles a
+            // wave step keep 4 at most for registers
+            // (one row tile a wave step: query fragments read a K step ahead,
+            // 3% faster on SQ8 10M x 384; two: no registers left for that)
+            const int nks = a.kdim * W::kElemBytes / 32;
+            if (a.wide_cfg == kWideCfg32R2)
+                return nks % 4 == 0 ? launch_wide32_t<W, 4, 2, 8, 4, false>(a, st)
+                                    : launch_wide32_t<W, 4, 2, 8, 2, false>(a, st);
+            if (nks % 8 == 0) return launch_wide32_t<W, 4, 1, 8, 8, true>(a, st);
+            if (nks % 6 == 0) return launch_wide32_t<W, 4, 1, 8, 6, true>(a, st);
+            if (nks % 4 == 0) return launch_wide32_t<W, 4, 1, 8, 4, true>(a, st);
+            return launch_wide32_t<W, 4, 1, 8, 2, true>(a, st);
+        }
+    }
     switch (a.wide_cfg) {
         case kWideCfgQ8R4W4:
             if (wide_fits<P>(8, a.kdim)) return launch_wide_t<P, 8, 4, 4, 1, true>(a, st);
@@ -918,6 +1396,11 @@ hipError_t launch_scan(const ScanArgs& a, hipStream_t st, const char** variant) 
 bool wide_cfg_fits(int kind, int kdim, int cfg) {
     const int qs = cfg == kWideCfgQ4R4W8 ? 4 : 8;
     if (cfg <= kWideCfgDefault || cfg >= kWideCfgCount) return false;
+    if (cfg == kWideCfg32R1 || cfg == kWideCfg32R2) {  // 128 queries, whole 64-byte row chunks
+        if (kind != SCAN_SQ8 && kind != SCAN_BF16) return false;
+        const int qbytes = kind == SCAN_SQ8 ? kdim : 2 * kdim;
+        return qbytes % 64 == 0 && (size_t)128 * (qbytes + 16 + 20) + 8 * 1024 <= 160 * 1024;
+    }
     switch (kind) {
         case SCAN_F32: return wide_fits<PolF32>(qs, kdim);
         case SCAN_BF16: return wide_fits<PolBF16>(qs, kdim);

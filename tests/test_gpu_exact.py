@@ -150,20 +150,22 @@ def test_large_batch_kernel_candidate_queue_paths(qcap, monkeypatch):
     np.testing.assert_array_equal(db, db0)
 
 
-@pytest.mark.parametrize("mode", ["exact", "bf16", "sq8"])
+@pytest.mark.parametrize("mode", ["exact", "bf16", "sq8", "sq4"])
 def test_large_batch_kernel_prefilter_modes(mode, monkeypatch):
-    """The wide kernel's epilogue prefilter (per-row bound, or one bound per
-    query when row norms are nearly equal) only decides which pairs take the
-    exact filter test: both modes give identical answers."""
+    """The wide kernel's epilogue prefilter (per-row bound; one bound per
+    query when row norms are nearly equal; for integer dots one bound per
+    query on dot minus each row's scaled norm excess) only decides which
+    pairs take the exact filter test: every mode gives identical answers."""
     base, Q = data.workload("exact-1m-384", n=150_000, n_queries=256)
     print(f"[case] wide prefilter modes {mode} n=150000 B=256", flush=True)
     out = []
-    for pf in ("1", "2"):
+    for pf in ("1", "2", "3"):
         monkeypatch.setenv("ANNMI_PREFILTER", pf)
         idx = Index(base, mode=mode, rerank=300)
         out.append(idx.search(Q, 100))
         del idx
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1], out[1][1])
+    for o in out[1:]:
+        np.testing.assert_array_equal(out[0][0], o[0])
+        np.testing.assert_array_equal(out[0][1], o[1])
     if mode == "exact":
         check_exact(base, Q[::64], 100, out[1][0][::64], out[1][1][::64])

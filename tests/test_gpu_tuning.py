@@ -27,7 +27,7 @@ def test_every_scan_shape_gives_the_same_answers(inputs, mode):
     ref_ids, ref_d = idx.search(Q, 100)
     if mode == "exact":
         np.testing.assert_array_equal(ref_ids, O.exact_topk_f64(base, Q, 100)[0])
-    shapes = [(1 << 30, 0, 0), (1, 0, 0), (1, 0, 4096)] + [(1, c, ch) for c in (1, 2, 3) for ch in (0, 8192)]
+    shapes = [(1 << 30, 0, 0), (1, 0, 0), (1, 0, 4096)] + [(1, c, ch) for c in (1, 2, 3, 4, 5) for ch in (0, 8192)]
     tried = 0
     for mb, cfg, ch in shapes:
         try:
