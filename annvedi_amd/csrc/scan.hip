@@ -276,26 +276,44 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                 }
             return;
         }
+        // Two phases: the tile's list reservations (one returning atomic per
+        // query and 16-row group with passing rows) are all issued before
+        // any result is used, so their round trips overlap instead of
+        // chaining (early ranges, where most pairs pass, waited on up to
+        // 4 * QW of them in a row per tile).
+        float dist[QW][4];
+        uint32_t gm[QW][4], base[QW][4];
+        uint32_t live = 0;  // bit 4w+i: some lane of the wave passes (wave-uniform)
 #pragma unroll
         for (int w = 0; w < QW; ++w) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float dot = (float)acc[w][i];
                 // dist = xn + qn - 2 * qscale * dot  (qscale folds the query/base scaling)
-                float dist = xn + qn[w][i] - 2.0f * qs[w][i] * dot;
-                bool pass = row_ok && dist <= thr[w][i];
-                uint64_t m = __ballot(pass);
-                if (m == 0) continue;
-                uint32_t gm = (uint32_t)(m >> (16 * g)) & 0xFFFFu;
-                int qi = q0 + 16 * w + 4 * g + i;
-                uint32_t base = 0;
-                if (n == 0 && gm) base = atomicAdd(&a.cand.cnt[qi], (uint32_t)__builtin_popcount(gm));
-                base = __shfl(base, lane & ~15);
-                if (pass) {
-                    uint32_t off = base + __builtin_popcount(gm & ((1u << n) - 1u));
+                dist[w][i] = xn + qn[w][i] - 2.0f * qs[w][i] * dot;
+                const bool pass = row_ok && dist[w][i] <= thr[w][i];
+                const uint64_t m = __ballot(pass);
+                gm[w][i] = (uint32_t)(m >> (16 * g)) & 0xFFFFu;
+                base[w][i] = 0;
+                if (m != 0) {
+                    live |= 1u << (4 * w + i);
+                    if (n == 0 && gm[w][i])
+                        base[w][i] = atomicAdd(&a.cand.cnt[q0 + 16 * w + 4 * g + i], (uint32_t)__builtin_popcount(gm[w][i]));
+                }
+            }
+        }
+        if (live == 0) return;
+#pragma unroll
+        for (int w = 0; w < QW; ++w) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (!((live >> (4 * w + i)) & 1u)) continue;
+                const uint32_t b = __shfl(base[w][i], lane & ~15);
+                if ((gm[w][i] >> n) & 1u) {
+                    const uint32_t off = b + __builtin_popcount(gm[w][i] & ((1u << n) - 1u));
                     if (off < (uint32_t)a.cand.cap) {
-                        int64_t slot = (int64_t)qi * a.cand.cap + off;
-                        a.cand.key[slot] = dist;
+                        const int64_t slot = (int64_t)(q0 + 16 * w + 4 * g + i) * a.cand.cap + off;
+                        a.cand.key[slot] = dist[w][i];
                         a.cand.rid[slot] = a.rid_map ? a.rid_map[row] : (uint32_t)row;
                     }
                 }
@@ -403,9 +421,16 @@ hipError_t launch_t(const ScanArgs& a, hipStream_t st) {
     const int qtiles = (a.B + qtile - 1) / qtile;
     const int64_t rows = a.row_end - a.row_begin;
     const int64_t tiles = (rows + 15) / 16;
-    // aim for ~8 workgroups per CU across all query tiles, >= 4 tiles per wave
+    // aim for ~8 workgroups per CU across all query tiles, 4 tiles a wave
+    // (its double-buffered stream runs across them), but at least 64
+    // workgroups a query tile down to one tile a wave: a small range (the
+    // first ones of a step) otherwise runs on a few workgroups whose tiles
+    // follow one another at load and list-reservation latency (1024 rows x
+    // 128 queries: 58 us on 8 workgroups, 21 us on 32)
     int64_t want = (2048 + qtiles - 1) / qtiles;
     int64_t maxb = (tiles + 15) / 16;
+    const int64_t spread = (tiles + 3) / 4 < 64 ? (tiles + 3) / 4 : 64;
+    if (maxb < spread) maxb = spread;
     int64_t gx = want < maxb ? want : maxb;
     if (gx < 1) gx = 1;
     size_t lds = (size_t)qtile * qrow_bytes<P>(a.kdim);
