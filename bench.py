@@ -262,16 +262,17 @@ def measured_traffic(workload: str, mode: str, kernel: str, launches: int):
     summarised in profiles/traffic_<workload>_<mode>.json (FETCH_SIZE and
     WRITE_SIZE over the timed launches of this same command, each scaled by
     the counter's reading for a copy of known size).  None when absent or
-    taken for another kernel, or for a launch count more than 10% away (the
-    autotuner's timing-based choices can move a few launches between kernel
-    instantiations from run to run)."""
+    taken for another kernel, or for a launch count more than 20% away (the
+    autotuner's timing-based choices move a few launches between kernel
+    instantiations from run to run: 63 to 72 of the dominant kernel's have
+    been seen on the default workload)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"traffic_{workload}_{mode}.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kernel or abs(int(t.get("launches", 0)) - launches) > 0.1 * max(launches, 1):
+    if t.get("kernel") != kernel or abs(int(t.get("launches", 0)) - launches) > 0.2 * max(launches, 1):
         return None
     return {"bytes_per_launch": int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]),
             "launches": int(t["launches"]),
