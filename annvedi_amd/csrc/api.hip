@@ -827,10 +827,13 @@ Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params*
         return nullptr;
     }
     {
-        // the scan stages one 16-query tile in LDS (64 KB per workgroup)
+        // the scan stages at least one 16-query tile in LDS (at most the 160
+        // KB of a CU); IVF also scans fp32 centroids (k-means, probe)
         const int kd = round_up(d, chunk_of(kind));
-        const int64_t lds = 16 * (scan_q_bytes(kind, kd) + 16);
-        if (lds > 65536) {
+        int64_t lds = 16 * (scan_q_bytes(kind, kd) + 16);
+        if (p.mode == ANNMI_IVF_SQ8)
+            lds = std::max<int64_t>(lds, 16 * (scan_q_bytes(SCAN_F32, round_up(d, chunk_of(SCAN_F32))) + 16));
+        if (lds > 160 * 1024) {
             set_err("dim %d too large for mode %d (query tile needs %lld bytes of LDS)", d, p.mode, (long long)lds);
             return nullptr;
         }

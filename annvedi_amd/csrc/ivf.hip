@@ -121,53 +121,95 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(IvfScanArgs a) {
             qs[i] = qe[i] >= 0 ? a.qscale[qe[i]] : 0.f;
         }
         const uint8_t* qrow = qa >= 0 ? static_cast<const uint8_t*>(a.Q) + (int64_t)qa * a.kdim : nullptr;
+        // rows of up to kRegChunks chunks (d <= 1024): the item's query
+        // fragments stay in registers across its tiles and every row chunk
+        // of a tile is requested before the first MFMA; longer rows take the
+        // chunk loop below
+        constexpr int kRegChunks = 16;
+        const bool reg = nchunk <= kRegChunks;
+        i32x4 qf[kRegChunks];
+        if (reg) {
+#pragma unroll
+            for (int c = 0; c < kRegChunks; ++c)
+                qf[c] = (qrow && c < nchunk) ? *reinterpret_cast<const i32x4*>(qrow + 16 * g + c * 64)
+                                             : (i32x4){0, 0, 0, 0};
+        }
         for (uint32_t r0 = item.row_begin; r0 < item.row_end; r0 += 16) {
             const uint32_t row = r0 + n;
             const bool rok = row < item.row_end;
             const int64_t lrow = rok ? row : item.row_end - 1;
             i32x4 acc = (i32x4){0, 0, 0, 0};
             const uint8_t* xr = X + lrow * a.xstride + 16 * g;
-            const uint8_t* qr = qrow ? qrow + 16 * g : nullptr;
-            // row and query fragments requested U chunks at a time before
-            // their MFMAs (the chunk count is a runtime value, so the
-            // compiler would otherwise wait on every load in turn)
-            constexpr int U = 4;
-            int c = 0;
-            for (; c + U <= nchunk; c += U) {
-                i32x4 b[U], q[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    b[u] = *reinterpret_cast<const i32x4*>(xr + (c + u) * 64);
-                    q[u] = qr ? *reinterpret_cast<const i32x4*>(qr + (c + u) * 64) : (i32x4){0, 0, 0, 0};
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const i32x4 bs = b[u] ^ (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
-                    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(q[u], bs, acc, 0, 0, 0);
-                }
-            }
-            for (; c < nchunk; ++c) {
-                i32x4 b = *reinterpret_cast<const i32x4*>(xr + c * 64);
-                b ^= (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
-                i32x4 q = qr ? *reinterpret_cast<const i32x4*>(qr + c * 64) : (i32x4){0, 0, 0, 0};
-                acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(q, b, acc, 0, 0, 0);
-            }
             const float xn = a.xn[lrow];
+            if (reg) {
+                i32x4 b[kRegChunks];
+#pragma unroll
+                for (int c = 0; c < kRegChunks; ++c)
+                    if (c < nchunk) b[c] = *reinterpret_cast<const i32x4*>(xr + c * 64);
+#pragma unroll
+                for (int c = 0; c < kRegChunks; ++c)
+                    if (c < nchunk) {
+                        const i32x4 bs =
+                            b[c] ^ (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
+                        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(qf[c], bs, acc, 0, 0, 0);
+                    }
+            } else {
+                const uint8_t* qr = qrow ? qrow + 16 * g : nullptr;
+                // row and query fragments requested U chunks at a time before
+                // their MFMAs (the chunk count is a runtime value, so the
+                // compiler would otherwise wait on every load in turn)
+                constexpr int U = 4;
+                int c = 0;
+                for (; c + U <= nchunk; c += U) {
+                    i32x4 b[U], q[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        b[u] = *reinterpret_cast<const i32x4*>(xr + (c + u) * 64);
+                        q[u] = qr ? *reinterpret_cast<const i32x4*>(qr + (c + u) * 64) : (i32x4){0, 0, 0, 0};
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const i32x4 bs =
+                            b[u] ^ (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
+                        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(q[u], bs, acc, 0, 0, 0);
+                    }
+                }
+                for (; c < nchunk; ++c) {
+                    i32x4 b = *reinterpret_cast<const i32x4*>(xr + c * 64);
+                    b ^= (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
+                    i32x4 q = qr ? *reinterpret_cast<const i32x4*>(qr + c * 64) : (i32x4){0, 0, 0, 0};
+                    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(q, b, acc, 0, 0, 0);
+                }
+            }
+            // Two phases: the tile's list reservations (one returning atomic
+            // per query and 16-row group with passing rows) are all issued
+            // before any result is used, so their round trips overlap (the
+            // first pass, with an open filter, appends every row)
+            float dist[4];
+            uint32_t gm[4], base[4];
+            uint32_t live = 0;  // bit i: some lane of the wave passes (wave-uniform)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float dist = xn + qn[i] - 2.0f * qs[i] * (float)acc[i];
-                const bool pass = rok && dist <= thr[i];
-                uint64_t mb = __ballot(pass);
-                if (mb == 0) continue;
-                uint32_t gm = (uint32_t)(mb >> (16 * g)) & 0xFFFFu;
-                uint32_t base = 0;
-                if (n == 0 && gm) base = atomicAdd(&a.cand.cnt[qe[i]], (uint32_t)__builtin_popcount(gm));
-                base = __shfl(base, lane & ~15);
-                if (pass) {
-                    uint32_t off = base + __builtin_popcount(gm & ((1u << n) - 1u));
+                dist[i] = xn + qn[i] - 2.0f * qs[i] * (float)acc[i];
+                const bool pass = rok && dist[i] <= thr[i];
+                const uint64_t mb = __ballot(pass);
+                gm[i] = (uint32_t)(mb >> (16 * g)) & 0xFFFFu;
+                base[i] = 0;
+                if (mb != 0) {
+                    live |= 1u << i;
+                    if (n == 0 && gm[i]) base[i] = atomicAdd(&a.cand.cnt[qe[i]], (uint32_t)__builtin_popcount(gm[i]));
+                }
+            }
+            if (live == 0) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (!((live >> i) & 1u)) continue;
+                const uint32_t bb = __shfl(base[i], lane & ~15);
+                if ((gm[i] >> n) & 1u) {
+                    const uint32_t off = bb + __builtin_popcount(gm[i] & ((1u << n) - 1u));
                     if (off < (uint32_t)a.cand.cap) {
-                        int64_t sl = (int64_t)qe[i] * a.cand.cap + off;
-                        a.cand.key[sl] = dist;
+                        const int64_t sl = (int64_t)qe[i] * a.cand.cap + off;
+                        a.cand.key[sl] = dist[i];
                         a.cand.rid[sl] = a.perm[row];
                     }
                 }

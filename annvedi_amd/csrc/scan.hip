@@ -434,6 +434,12 @@ hipError_t launch_t(const ScanArgs& a, hipStream_t st) {
     int64_t gx = want < maxb ? want : maxb;
     if (gx < 1) gx = 1;
     size_t lds = (size_t)qtile * qrow_bytes<P>(a.kdim);
+    static bool attr = false;
+    if (lds > 65536 && !attr) {  // long rows (one 16-query subtile): up to the 160 KB of LDS
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_kernel<P, QW>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
     hipLaunchKernelGGL((scan_kernel<P, QW>), dim3((unsigned)gx, (unsigned)qtiles), dim3(kThreads), lds, st, a);
     return hipGetLastError();
 }
@@ -1380,13 +1386,15 @@ hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
 }
 
 // Largest query tile (16, 32 or 64 queries) that the batch can use and
-// whose LDS staging stays within 64 KB.
+// whose LDS staging stays within 64 KB (several workgroups a CU); rows too
+// long for that take 16-query tiles of up to the 160 KB of LDS (fp32 to
+// d = 2556).
 template <class P>
 hipError_t launch_p(const ScanArgs& a, hipStream_t st) {
     const size_t row = (size_t)qrow_bytes<P>(a.kdim);
     if (a.B > 32 && 64 * row <= 65536) return launch_t<P, 4>(a, st);
     if (a.B > 16 && 32 * row <= 65536) return launch_t<P, 2>(a, st);
-    if (16 * row <= 65536) return launch_t<P, 1>(a, st);
+    if (16 * row <= 160 * 1024) return launch_t<P, 1>(a, st);
     return hipErrorInvalidValue;
 }
 

@@ -9,7 +9,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(1, 8), (17, 3), (300, 1), (300, 129), (2049, 40)]
+SHAPES = [(1, 8), (17, 3), (300, 1), (300, 129), (2049, 40), (500, 1100)]
 
 
 @pytest.mark.parametrize("mode", ["exact", "bf16", "fp8", "sq8", "sq4"])

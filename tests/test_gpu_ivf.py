@@ -51,8 +51,12 @@ def test_kmeans_first_round_follows_reference_init():
             np.testing.assert_array_equal(cent[l], base[ri[(l * 37) % 20000]])
 
 
-def test_ivf_full_rescore_equals_bruteforce_over_probed_lists():
-    base, Q = zipf_data(16000, 64, 32, 24, seed=1)
+@pytest.mark.parametrize("d", [64, 768, 1100])
+def test_ivf_full_rescore_equals_bruteforce_over_probed_lists(d):
+    """d = 768: 12 code chunks a row (query fragments held in registers);
+    d = 1100: 18 chunks (the list scan's chunk loop)."""
+    print(f"[case] ivf brute force d={d}", flush=True)
+    base, Q = zipf_data(16000, d, 32, 24, seed=1)
     idx = Index(base, mode="ivf", nlist=32, kmeans_iters=4)
     perm, off = lists_of(idx)
     cent = idx.export("centroids")
