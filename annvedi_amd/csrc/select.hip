@@ -19,24 +19,96 @@ namespace {
 
 constexpr int kRMax = 2048;   // most candidates one query re-scores
 
+// Lists of up to kStageKeys candidates are staged in LDS as ordered keys
+// (eight loads a lane in flight) and selected there: a block reading the
+// list from global memory once per radix pass waited on load latency for
+// most of its time.  After the threshold the list is compacted in place to
+// the candidates at or below it: the threshold only falls from range to
+// range and finalize collects keys at or below the final rank-rsel key plus
+// the same slack, so no dropped candidate could be collected later, and the
+// next selections and finalize see ~rsel * rho keys instead of everything
+// appended so far.
+constexpr int kStageKeys = 8192;
+
 __global__ __launch_bounds__(kThreads) void select_threshold(SelectArgs a) {
     __shared__ uint32_t hist[kBins];
     __shared__ uint32_t sh[8];
+    __shared__ uint32_t wsum[kThreads / 64];
+    __shared__ float tsh;
+    extern __shared__ uint32_t skeys[];  // [kStageKeys]
     const int q = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t cnt = a.cand.cnt[q];
     if (cnt > (uint32_t)a.cand.cap) {
         if (threadIdx.x == 0) a.overflow[q] |= 1u;
         cnt = (uint32_t)a.cand.cap;
     }
     if (cnt < (uint32_t)a.rsel) return;  // not enough rows seen yet: keep +inf
-    uint32_t k = radix_select(a.cand.key + (int64_t)q * a.cand.cap, (int)cnt, (uint32_t)a.rsel, hist, sh);
+    float* keys = a.cand.key + (int64_t)q * a.cand.cap;
+    uint32_t* rids = a.cand.rid + (int64_t)q * a.cand.cap;
+    const bool staged = cnt <= (uint32_t)kStageKeys;
+    uint32_t k;
+    if (staged) {
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < (int)cnt; i0 += kThreads * U) {
+            float v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * kThreads + (int)threadIdx.x;
+                v[u] = i < (int)cnt ? keys[i] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * kThreads + (int)threadIdx.x;
+                if (i < (int)cnt) skeys[i] = f2key(v[u]);
+            }
+        }
+        __syncthreads();
+        k = radix_select_lds(skeys, (int)cnt, (uint32_t)a.rsel, hist, sh);
+    } else {
+        k = radix_select(keys, (int)cnt, (uint32_t)a.rsel, hist, sh);
+    }
     if (threadIdx.x == 0) {
         float t = key2f(k) + a.slack[q];
         // round the threshold up by one ulp so that the fp32 add never cuts a
         // candidate whose exact sum would be admitted
         t = next_up(t);
         if (t < a.thr[q]) a.thr[q] = t;
+        tsh = a.thr[q];
     }
+    __syncthreads();
+    // compaction in place, order kept: a chunk's reads all happen before its
+    // writes, and a kept entry moves to an index no larger than its own
+    const float t = tsh;
+    uint32_t base = 0;
+    for (int i0 = 0; i0 < (int)cnt; i0 += kThreads) {
+        const int i = i0 + (int)threadIdx.x;
+        float kf = 0.f;
+        uint32_t rid = 0;
+        bool keep = false;
+        if (i < (int)cnt) {
+            kf = staged ? key2f(skeys[i]) : keys[i];
+            keep = kf <= t;
+            if (keep) rid = rids[i];
+        }
+        const uint64_t m = __ballot(keep);
+        if (lane == 0) wsum[wave] = (uint32_t)__builtin_popcountll(m);
+        __syncthreads();
+        uint32_t off = base, total = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) {
+            if (w < wave) off += wsum[w];
+            total += wsum[w];
+        }
+        off += (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+        if (keep) {
+            keys[off] = kf;
+            rids[off] = rid;
+        }
+        base += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.cand.cnt[q] = base;
 }
 
 // ---------------------------------------------------------------- finalize
@@ -264,7 +336,7 @@ __global__ __launch_bounds__(kThreads) void finalize(FinalArgs a) {
 
 hipError_t launch_select_threshold(const SelectArgs& a, hipStream_t st) {
     if (a.B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(select_threshold, dim3((unsigned)a.B), dim3(kThreads), 0, st, a);
+    hipLaunchKernelGGL(select_threshold, dim3((unsigned)a.B), dim3(kThreads), kStageKeys * sizeof(uint32_t), st, a);
     return hipGetLastError();
 }
 

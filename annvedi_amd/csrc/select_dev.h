@@ -84,6 +84,31 @@ __device__ uint32_t radix_select(const float* keys, int n, uint32_t rank, uint32
     return prefix;
 }
 
+// Key of rank `rank` (1-based) among n ordered keys already in LDS.
+__device__ uint32_t radix_select_lds(const uint32_t* keys, int n, uint32_t rank, uint32_t* hist, uint32_t* sh) {
+    uint32_t prefix = 0, prefix_mask = 0;
+    const int shifts[3] = {21, 10, 0};
+    const int widths[3] = {11, 11, 10};
+    for (int p = 0; p < 3; ++p) {
+        for (int b = threadIdx.x; b < kBins; b += kThreads) hist[b] = 0;
+        __syncthreads();
+        const int sft = shifts[p];
+        const uint32_t wmask = (1u << widths[p]) - 1u;
+        for (int i = threadIdx.x; i < n; i += kThreads) {
+            const uint32_t k = keys[i];
+            if ((k & prefix_mask) == prefix) atomicAdd(&hist[(k >> sft) & wmask], 1u);
+        }
+        __syncthreads();
+        int bin;
+        uint32_t below;
+        find_bin(hist, rank, sh, &bin, &below);
+        rank -= below;
+        prefix |= (uint32_t)bin << sft;
+        prefix_mask |= wmask << sft;
+    }
+    return prefix;
+}
+
 // Bitonic sort of n (power of two) 64-bit keys in LDS, ascending.
 __device__ void bitonic_sort(uint64_t* v, int n) {
     for (int size = 2; size <= n; size <<= 1) {
