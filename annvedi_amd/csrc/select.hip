@@ -227,10 +227,33 @@ __global__ __launch_bounds__(kThreads) void finalize(FinalArgs a) {
     const float* keys = a.cand.key + (int64_t)q * a.cand.cap;
     const uint32_t* rids = a.cand.rid + (int64_t)q * a.cand.cap;
 
+    // Lists of up to 2 * kRMax keys (the usual case: the range selections
+    // compact them) are staged once in the sort buffer, free until step 4,
+    // as ordered keys for the selection and the collection.
+    uint32_t* skeys = reinterpret_cast<uint32_t*>(S.sk);
+    const bool staged = cnt <= (uint32_t)(2 * kRMax);
+    if (staged) {
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < (int)cnt; i0 += kThreads * U) {
+            float v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * kThreads + (int)threadIdx.x;
+                v[u] = i < (int)cnt ? keys[i] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * kThreads + (int)threadIdx.x;
+                if (i < (int)cnt) skeys[i] = f2key(v[u]);
+            }
+        }
+        __syncthreads();
+    }
     // 1. collection threshold
     float thr = __builtin_inff();
     if (cnt > (uint32_t)a.rsel) {
-        uint32_t k = radix_select(keys, (int)cnt, (uint32_t)a.rsel, S.hist, S.sh);
+        uint32_t k = staged ? radix_select_lds(skeys, (int)cnt, (uint32_t)a.rsel, S.hist, S.sh)
+                            : radix_select(keys, (int)cnt, (uint32_t)a.rsel, S.hist, S.sh);
         thr = key2f(k);
         if (a.slack[q] > 0.f) thr = next_up(thr + a.slack[q]);
     }
@@ -238,7 +261,7 @@ __global__ __launch_bounds__(kThreads) void finalize(FinalArgs a) {
     if (threadIdx.x == 0) S.ncol = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < (int)cnt; i += kThreads) {
-        if (keys[i] <= thr) {
+        if ((staged ? key2f(skeys[i]) : keys[i]) <= thr) {
             uint32_t pos = atomicAdd(&S.ncol, 1u);
             if (pos < (uint32_t)kRMax) S.id[pos] = rids[i];
         }
