@@ -468,8 +468,12 @@ struct WideQueue {
     uint32_t* row;  // (row - chunk begin) << 7 | query slot in the tile
     float* key;
     uint32_t* count;  // LDS counter of this wave
+    uint32_t* hist;   // [128] per-slot counts, then next list slots, of a flush
     int cap;
 };
+
+// LDS words a wave's flush histogram takes (one per query slot of a tile)
+constexpr int kFlushSlots = 128;
 
 // Append one pair (lanes diverge: only passing lanes get here).
 __device__ inline void wide_append(const ScanArgs& a, const WideQueue& wq, int qi, int m, uint32_t roff,
@@ -488,32 +492,51 @@ __device__ inline void wide_append(const ScanArgs& a, const WideQueue& wq, int q
     }
 }
 
-// Move the wave's queued pairs to the per-query lists: one returning global
-// atomic per entry, eight per lane issued before a wait.
-__device__ inline void wide_flush(const ScanArgs& a, const WideQueue& wq, int q0, int64_t r_begin, int lane) {
+// Move the wave's queued pairs to the per-query lists, aggregated per query:
+// count the entries of each query slot in LDS, reserve each query's space
+// with one global atomic (at most 128 a flush, not one per entry: a range
+// appending ~R * rho pairs a query spent about half of its first wide launch
+// on those atomics), then scatter with LDS offsets.  A list's entries are a
+// set; the order in which a flush writes them does not matter.
+__device__ inline void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline void wide_flush(const ScanArgs& a, const WideQueue& wq, int q0, int64_t r_begin, int lane) {
+    wave_sync_lds();
     uint32_t cnt = __builtin_amdgcn_readfirstlane(*wq.count);
     const int count = cnt < (uint32_t)wq.cap ? (int)cnt : wq.cap;
-    constexpr int U = 8;
-    for (int e0 = 0; e0 < count; e0 += 64 * U) {
-        uint32_t v[U], off[U];
-#pragma unroll
-        for (int j = 0; j < U; ++j) {
-            const int e = e0 + 64 * j + lane;
-            v[j] = e < count ? wq.row[e] : 0u;
-            off[j] = e < count ? atomicAdd(&a.cand.cnt[q0 + (int)(v[j] & 127u)], 1u) : ~0u;
+    if (count > 0) {
+        for (int m = lane; m < kFlushSlots; m += 64) wq.hist[m] = 0u;
+        wave_sync_lds();
+        for (int e = lane; e < count; e += 64) atomicAdd(&wq.hist[wq.row[e] & 127u], 1u);
+        wave_sync_lds();
+        for (int m = lane; m < kFlushSlots; m += 64) {
+            const uint32_t c = wq.hist[m];
+            wq.hist[m] = c ? atomicAdd(&a.cand.cnt[q0 + m], c) : 0u;
         }
+        wave_sync_lds();
+        constexpr int U = 4;
+        for (int e0 = 0; e0 < count; e0 += 64 * U) {
+            uint32_t v[U], off[U];
 #pragma unroll
-        for (int j = 0; j < U; ++j) {
-            const int e = e0 + 64 * j + lane;
-            if (off[j] < (uint32_t)a.cand.cap) {
-                const int qi = q0 + (int)(v[j] & 127u);
-                const uint32_t row = (uint32_t)(r_begin + (v[j] >> 7));
-                const int64_t sl = (int64_t)qi * a.cand.cap + off[j];
-                a.cand.key[sl] = wq.key[e];
-                a.cand.rid[sl] = a.rid_map ? a.rid_map[row] : row;
+            for (int j = 0; j < U; ++j) {
+                const int e = e0 + 64 * j + lane;
+                v[j] = e < count ? wq.row[e] : 0u;
+                off[j] = e < count ? atomicAdd(&wq.hist[v[j] & 127u], 1u) : ~0u;
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const int e = e0 + 64 * j + lane;
+                if (off[j] < (uint32_t)a.cand.cap) {
+                    const int qi = q0 + (int)(v[j] & 127u);
+                    const uint32_t row = (uint32_t)(r_begin + (v[j] >> 7));
+                    const int64_t sl = (int64_t)qi * a.cand.cap + off[j];
+                    a.cand.key[sl] = wq.key[e];
+                    a.cand.rid[sl] = a.rid_map ? a.rid_map[row] : row;
+                }
             }
         }
     }
@@ -674,6 +697,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     wq.row = L.cnt + kWideWaves + (int64_t)wave * 2 * qcap;
     wq.key = reinterpret_cast<float*>(wq.row + qcap);
     wq.count = L.cnt + wave;
+    wq.hist = L.cnt + kWideWaves + (int64_t)kWideWaves * 2 * qcap + wave * kFlushSlots;
     wq.cap = qcap;
     constexpr bool kIntDot = std::is_same<El, int>::value;
     const float kap = kIntDot ? wide_kap(a, L, QT, lane) : 0.f;
@@ -953,9 +977,11 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     chunk = (chunk + step - 1) / step * step;
     const int64_t nrc = (rows + chunk - 1) / chunk;
     const int64_t per = (nrc + 7) / 8;
-    // the rest of the LDS holds the waves' candidate queues (8 bytes an
-    // entry); below 1024 entries a wave, every pair goes straight to the lists
-    int qcap = (int)((160 * 1024 - lds) / (NW * 8)) / 64 * 64;
+    // the rest of the LDS holds the waves' flush histograms and candidate
+    // queues (8 bytes an entry); below 1024 entries a wave, every pair goes
+    // straight to the lists
+    const size_t hist_bytes = (size_t)NW * kFlushSlots * sizeof(uint32_t);
+    int qcap = (int)((160 * 1024 - lds - hist_bytes) / (NW * 8)) / 64 * 64;
     if (qcap > 4096) qcap = 4096;
     if (const char* e = getenv("ANNMI_WIDE_QCAP")) {  // lower cap: tuning and tests
         const int v = atoi(e);
@@ -963,7 +989,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     }
     if (qcap < 1024) qcap = 0;
     hipLaunchKernelGGL((scan_wide_kernel<P, QS, RS, NW, QH, QPRE>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
-                       lds + (size_t)NW * qcap * 8, st, a, qtiles, chunk, nrc, qcap);
+                       lds + (size_t)NW * qcap * 8 + hist_bytes, st, a, qtiles, chunk, nrc, qcap);
     return hipGetLastError();
 }
 
@@ -1046,6 +1072,7 @@ __global__ __launch_bounds__(64 * kWaves) void scan_wide32_kernel(ScanArgs a, in
     wq.row = L.cnt + kWaves + (int64_t)wave * 2 * qcap;
     wq.key = reinterpret_cast<float*>(wq.row + qcap);
     wq.count = L.cnt + wave;
+    wq.hist = L.cnt + kWaves + (int64_t)kWaves * 2 * qcap + wave * kFlushSlots;
     wq.cap = qcap;
     const float kap = kIntDot ? wide_kap(a, L, QT, lane) : 0.f;
 
@@ -1302,7 +1329,10 @@ hipError_t launch_wide32_t(const ScanArgs& a, hipStream_t st) {
     chunk = (chunk + step - 1) / step * step;
     const int64_t nrc = (rows + chunk - 1) / chunk;
     const int64_t per = (nrc + 7) / 8;
-    int qcap = (int)((160 * 1024 - lds) / (NW * 8)) / 64 * 64;
+    // the rest of the LDS holds the waves' flush histograms and candidate
+    // queues (8 bytes an entry)
+    const size_t hist_bytes = (size_t)NW * kFlushSlots * sizeof(uint32_t);
+    int qcap = (int)((160 * 1024 - lds - hist_bytes) / (NW * 8)) / 64 * 64;
     if (qcap > 4096) qcap = 4096;
     if (const char* e = getenv("ANNMI_WIDE_QCAP")) {  // lower cap: tuning and tests
         const int v = atoi(e);
@@ -1310,7 +1340,7 @@ hipError_t launch_wide32_t(const ScanArgs& a, hipStream_t st) {
     }
     if (qcap < 1024) qcap = 0;
     hipLaunchKernelGGL((scan_wide32_kernel<W, QS, RS, NW, D, QP>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
-                       lds + (size_t)NW * qcap * 8, st, a, qtiles, chunk, nrc, qcap);
+                       lds + (size_t)NW * qcap * 8 + hist_bytes, st, a, qtiles, chunk, nrc, qcap);
     return hipGetLastError();
 }
 
