@@ -182,6 +182,14 @@ def hnsw_baseline(base_t, Q_t, n_s=200_000, nq=200):
                       f"time; QPS not scaled (HNSW search cost grows ~log n)"}
 
 
+def workload_shape(workload: str, n: int = 0, d: int = 0):
+    """(rows, dims) of a named workload; n or d > 0 overrides it."""
+    from annvedi_amd import data
+
+    wn, wd = data.WORKLOADS[workload][:2]
+    return (n or wn), (d or wd)
+
+
 def bench_ivf(args):
     """Served IVF + SQ8: 10M x 768 Zipf(1.1) mixture,
     k-means nlist = 4096, nprobe in {16, 64}, Poisson arrivals, request sizes
@@ -193,7 +201,8 @@ def bench_ivf(args):
     from annvedi_amd.serving import Server, lognormal_sizes, run_open_loop
 
     dev = "cuda:0"
-    n, d, k = args.n, 768, args.k
+    n, d = workload_shape("ivf-10m-768", args.n, args.d)
+    k = args.k
     log(f"[ivf] generating {n} x {d} Zipf mixture in HBM")
     base = data.mixture_torch(n, d, 4096, 0.3, 2004, center_seed=1004, normalize=False, zipf_s=1.1, device=dev)
     sizes = lognormal_sizes(args.requests, seed=4002)
@@ -322,9 +331,10 @@ def main():
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
-    wn, wd, C, sigma, norm, dscale, zs = data.WORKLOADS[args.workload]
+    _, _, C, sigma, norm, dscale, zs = data.WORKLOADS[args.workload]
     cid = {"exact-1m-384": 1, "sq-1m-768": 2, "mixed-10m-384": 3}[args.workload]  # seeds 1000+c, 2000+c, 3000+c
-    n, d, k = args.n or wn, args.d or wd, args.k
+    n, d = workload_shape(args.workload, args.n, args.d)
+    k = args.k
     gen = dict(center_seed=data.SEED_CENTERS + cid, normalize=norm, dim_scale=dscale, zipf_s=zs, device=dev)
     log(f"[bench] generating {n} x {d} mixture in HBM")
     base = data.mixture_torch(n, d, C, sigma, data.SEED_BASE + cid, **gen)

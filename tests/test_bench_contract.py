@@ -28,6 +28,13 @@ def test_batch_mix_is_fixed_and_log_uniform():
     assert counts.min() > 0.6 * counts.mean()
 
 
+def test_workload_shapes_default_to_the_named_workload():
+    assert bench.workload_shape("mixed-10m-384") == (10_000_000, 384)
+    assert bench.workload_shape("ivf-10m-768") == (10_000_000, 768)
+    assert bench.workload_shape("sq-1m-768", 5000, 0) == (5000, 768)
+    assert bench.workload_shape("exact-1m-384", 0, 96) == (1_000_000, 96)
+
+
 def test_per_launch_roofline():
     # one HBM-bound launch at exactly the HBM peak, one MFMA-bound at half
     ms = [1.0, 2.0]
