@@ -963,7 +963,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     }
     const int qtiles = (a.B + QT - 1) / QT;
     const int64_t rows = a.row_end - a.row_begin;
-    // row chunk: about 8 workgroups per CU over the launch, one wave step
+    // row chunk: a few workgroups per CU over the launch, one wave step
     // (16 * RS rows per wave) to 32K rows, whole wave steps
     constexpr int64_t step = 16 * RS * (NW / QH);
     static const int64_t chunk_env = [] {  // tuning: ANNMI_WIDE_CHUNK_MAX rows
@@ -972,7 +972,13 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
         return v >= 2048 ? (int64_t)v : (int64_t)32768;
     }();
     const int64_t chunk_max = a.chunk_max >= 2048 ? a.chunk_max : chunk_env;
-    int64_t chunk = rows * qtiles / ((int64_t)ncu * 8);
+    // workgroups per CU over the launch: up to 8, but no more than the query
+    // tiles, since each workgroup stages its tile and with few tiles the
+    // staging is not shared by many workgroups of a row chunk (128 queries
+    // over 10M x 384 SQ8: 1.166 -> 1.084 ms of scan a step with one per CU;
+    // 512 queries best at 4, 4096 at 4-8)
+    const int64_t wg_per_cu = qtiles < 8 ? qtiles : 8;
+    int64_t chunk = rows * qtiles / ((int64_t)ncu * wg_per_cu);
     chunk = chunk < step ? step : (chunk > chunk_max ? chunk_max : chunk);
     chunk = (chunk + step - 1) / step * step;
     const int64_t nrc = (rows + chunk - 1) / chunk;
@@ -1315,8 +1321,9 @@ hipError_t launch_wide32_t(const ScanArgs& a, hipStream_t st) {
     }
     const int qtiles = (a.B + QT - 1) / QT;
     const int64_t rows = a.row_end - a.row_begin;
-    // row chunk as for the 16 x 16 kernel: about 8 workgroups per CU over
-    // the launch, whole wave steps, at most 32K rows (or the tuned maximum)
+    // row chunk as for the 16 x 16 kernel: up to 8 workgroups per CU over
+    // the launch (no more than the query tiles), whole wave steps, at most
+    // 32K rows (or the tuned maximum)
     constexpr int64_t step = 32 * RS * NW;
     static const int64_t chunk_env = [] {
         const char* e = getenv("ANNMI_WIDE_CHUNK_MAX");
@@ -1324,7 +1331,13 @@ hipError_t launch_wide32_t(const ScanArgs& a, hipStream_t st) {
         return v >= 2048 ? (int64_t)v : (int64_t)32768;
     }();
     const int64_t chunk_max = a.chunk_max >= 2048 ? a.chunk_max : chunk_env;
-    int64_t chunk = rows * qtiles / ((int64_t)ncu * 8);
+    // workgroups per CU over the launch: up to 8, but no more than the query
+    // tiles, since each workgroup stages its tile and with few tiles the
+    // staging is not shared by many workgroups of a row chunk (128 queries
+    // over 10M x 384 SQ8: 1.166 -> 1.084 ms of scan a step with one per CU;
+    // 512 queries best at 4, 4096 at 4-8)
+    const int64_t wg_per_cu = qtiles < 8 ? qtiles : 8;
+    int64_t chunk = rows * qtiles / ((int64_t)ncu * wg_per_cu);
     chunk = chunk < step ? step : (chunk > chunk_max ? chunk_max : chunk);
     chunk = (chunk + step - 1) / step * step;
     const int64_t nrc = (rows + chunk - 1) / chunk;

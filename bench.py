@@ -331,7 +331,7 @@ def main():
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
-    _, _, C, sigma, norm, dscale, zs = data.WORKLOADS[args.workload]
+    wn, wd, C, sigma, norm, dscale, zs = data.WORKLOADS[args.workload]
     cid = {"exact-1m-384": 1, "sq-1m-768": 2, "mixed-10m-384": 3}[args.workload]  # seeds 1000+c, 2000+c, 3000+c
     n, d = workload_shape(args.workload, args.n, args.d)
     k = args.k

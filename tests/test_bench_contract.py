@@ -28,6 +28,29 @@ def test_batch_mix_is_fixed_and_log_uniform():
     assert counts.min() > 0.6 * counts.mean()
 
 
+def test_bench_script_names_only_defined_names():
+    """A name bench.py loads but never binds (a renamed variable) shows up
+    only when the GPU run reaches that line; catch it here."""
+    import ast
+    import builtins
+
+    tree = ast.parse(open(os.path.join(ROOT, "bench.py")).read())
+    bound = {"__file__", "__name__"}
+    for n in ast.walk(tree):
+        if isinstance(n, ast.Name) and isinstance(n.ctx, (ast.Store, ast.Del)):
+            bound.add(n.id)
+        elif isinstance(n, (ast.FunctionDef, ast.ClassDef)):
+            bound.add(n.name)
+        elif isinstance(n, ast.arg):
+            bound.add(n.arg)
+        elif isinstance(n, (ast.Import, ast.ImportFrom)):
+            bound.update((a.asname or a.name).split(".")[0] for a in n.names)
+        elif isinstance(n, ast.ExceptHandler) and n.name:
+            bound.add(n.name)
+    used = {n.id for n in ast.walk(tree) if isinstance(n, ast.Name) and isinstance(n.ctx, ast.Load)}
+    assert sorted(u for u in used - bound if not hasattr(builtins, u)) == []
+
+
 def test_workload_shapes_default_to_the_named_workload():
     assert bench.workload_shape("mixed-10m-384") == (10_000_000, 384)
     assert bench.workload_shape("ivf-10m-768") == (10_000_000, 768)
