@@ -1560,6 +1560,20 @@ bool autotune_impl(Handle* h, int max_batch, int reps) {
                 }
             }
         }
+        // confirm a switch: time the built-in shape and the winner once more
+        // each and keep the winner only if its best time still beats the
+        // built-in one's by 2% (single timings of a step vary by a few
+        // percent, which otherwise decides some picks)
+        if (ok && best.kernel != -1) {
+            double t_def2 = 0, t_best2 = 0;
+            if (!(ok = time_step(B, def, t_def2)) || !(ok = time_step(B, best, t_best2))) break;
+            t_def = std::min(t_def, t_def2);
+            t_best = std::min(t_best, t_best2);
+            if (!(t_best < 0.98 * t_def)) {
+                best = def;
+                t_best = t_def;
+            }
+        }
         if (!ok) break;
         table.push_back(best);
         char line[200];
