@@ -266,24 +266,29 @@ def copy_calibration(gib: float):
     print(json.dumps({"copy_bytes": nbytes}))
 
 
-def measured_traffic(workload: str, mode: str, kernel: str, launches: int):
-    """HBM bytes per launch of the dominant kernel, from the counter runs
-    summarised in profiles/traffic_<workload>_<mode>.json (FETCH_SIZE and
+def measured_traffic(workload: str, mode: str, kernel: str, alg_bytes_per_launch: float):
+    """HBM bytes per launch of the dominant kernel from the counter runs
+    summarised in profiles/traffic_<workload>_<mode>.json: FETCH_SIZE and
     WRITE_SIZE over the timed launches of this same command, each scaled by
-    the counter's reading for a copy of known size).  None when absent or
-    taken for another kernel, or for a launch count more than 20% away (the
-    autotuner's timing-based choices move a few launches between kernel
-    instantiations from run to run: 63 to 72 of the dominant kernel's have
-    been seen on the default workload)."""
+    the counter's reading for a copy of known size and divided by its own
+    run's algorithmic bytes per launch.  Those ratios times this run's
+    algorithmic bytes per launch give the estimate: the autotuner's
+    timing-based choices move whole batch sizes between kernel
+    instantiations from run to run (61 to 81 timed launches of the dominant
+    kernel on the default workload), which moves the per-launch average but
+    not the bytes fetched per algorithmic byte.  None when absent or taken
+    for another kernel."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"traffic_{workload}_{mode}.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kernel or abs(int(t.get("launches", 0)) - launches) > 0.2 * max(launches, 1):
+    if t.get("kernel") != kernel:
         return None
-    return {"bytes_per_launch": int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]),
+    ratio = float(t["fetch_ratio"]) + float(t["write_ratio"])
+    return {"bytes_per_launch": int(round(ratio * alg_bytes_per_launch)),
+            "ratio_to_algorithmic": round(ratio, 4),
             "launches": int(t["launches"]),
             "source": os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))}
 
@@ -437,12 +442,12 @@ def main():
     roof["traffic"] = None
     roof["algorithmic_bytes_per_launch"] = int(top["bytes"] / max(top["launches"], 1))
     wname = args.workload if (n, d) == (wn, wd) else f"{args.workload}-n{n}-d{d}"
-    tr = measured_traffic(wname, mode, top["kernel"],
-                          int(top["launches"]))
+    tr = measured_traffic(wname, mode, top["kernel"], roof["algorithmic_bytes_per_launch"])
     if tr is not None:
         roof["traffic"] = tr["bytes_per_launch"]
+        roof["traffic_ratio_to_algorithmic"] = tr["ratio_to_algorithmic"]
         roof["traffic_source"] = tr["source"]
-        roof["traffic_launches"] = tr["launches"]
+        roof["traffic_counter_launches"] = tr["launches"]
     roof["kernel"] = top["kernel"]
     roof["launches"] = int(top["launches"])
     roof["avg_launch_ms"] = round(top["ms"] / max(top["launches"], 1), 4)

@@ -68,7 +68,7 @@ def test_per_launch_roofline():
     assert bench.per_launch_roofline([], [], [], 5000.0) is None
 
 
-def test_traffic_profiles_are_consistent_and_matched_by_launch_count():
+def test_traffic_profiles_are_consistent_and_scaled_by_algorithmic_bytes():
     for mode in ("sq8", "exact"):
         path = os.path.join(ROOT, "profiles", f"traffic_mixed-10m-384_{mode}.json")
         t = json.load(open(path))
@@ -77,9 +77,14 @@ def test_traffic_profiles_are_consistent_and_matched_by_launch_count():
             t["fetch_bytes_per_launch"]
         assert abs(t["write_bytes_per_launch"] - t["write_kib_per_launch"] * 1024 * cal["write_scale"]) < 1e-4 * \
             max(t["write_bytes_per_launch"], 1)
+        assert abs(t["fetch_ratio"] - t["fetch_bytes_per_launch"] / t["fetch_run_algorithmic_bytes_per_launch"]) < 1e-4
+        assert abs(t["write_ratio"] - t["write_bytes_per_launch"] / t["write_run_algorithmic_bytes_per_launch"]) < 1e-4
+        assert 0.9 < t["fetch_ratio"] < 2.0  # each row read at least about once, and not wildly more
         assert os.path.exists(os.path.join(ROOT, t["summary"]))
-        n = t["launches"]
-        hit = bench.measured_traffic("mixed-10m-384", mode, t["kernel"], n)
-        assert hit["bytes_per_launch"] == int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"])
-        assert bench.measured_traffic("mixed-10m-384", mode, t["kernel"], int(n * 1.5)) is None
-        assert bench.measured_traffic("mixed-10m-384", mode, "another_kernel", n) is None
+        alg = t["fetch_run_algorithmic_bytes_per_launch"]
+        hit = bench.measured_traffic("mixed-10m-384", mode, t["kernel"], alg)
+        want = (t["fetch_ratio"] + t["write_ratio"]) * alg
+        assert abs(hit["bytes_per_launch"] - want) <= 1
+        # scales with this run's algorithmic bytes; another kernel gets none
+        assert abs(bench.measured_traffic("mixed-10m-384", mode, t["kernel"], 2 * alg)["bytes_per_launch"] - 2 * want) <= 2
+        assert bench.measured_traffic("mixed-10m-384", mode, "another_kernel", alg) is None
