@@ -1431,7 +1431,7 @@ hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
 // Largest query tile (16, 32 or 64 queries) that the batch can use and
 // whose LDS staging stays within 64 KB (several workgroups a CU); rows too
 // long for that take 16-query tiles of up to the 160 KB of LDS (fp32 to
-// d = 2556).
+// d = 2544).
 template <class P>
 hipError_t launch_p(const ScanArgs& a, hipStream_t st) {
     const size_t row = (size_t)qrow_bytes<P>(a.kdim);

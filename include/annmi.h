@@ -50,7 +50,11 @@ void annmi_default_params(annmi_params* p, int32_t mode);
  * Replaces: LinearIndex::fit        src/linear_index.cpp:73-88
  *           gf_create               competitors/gpu_filter/src/gpu_filter.cu:79-140
  * Returns NULL on error (annmi_last_error()).  A host `base` is copied; a
- * device `base` (params->base_on_device) is borrowed when d % 16 == 0. */
+ * device `base` (params->base_on_device) is borrowed when d % 16 == 0.
+ * d is limited by the LDS a 16-query tile of the scan takes (160 KB): fp32
+ * (exact mode, and IVF, whose k-means and probe scan fp32 centroids) up to
+ * d = 2544; the other modes up to 4096 (the library's maximum).  Larger d
+ * fails here with an error naming the limit. */
 void* annmi_create(const float* base, int64_t n, int32_t d, const annmi_params* params);
 
 /* One query (host pointers).  Writes min(k, n) ids and distances.
@@ -163,8 +167,9 @@ void* annmi_load(const float* base, int64_t n, int32_t d, const annmi_params* pa
  * queries drawn from the index's own rows: the streaming kernel (up to 128
  * queries), each wide-kernel tile configuration that fits the LDS, the
  * built-in one at other row-chunk sizes, then the best one at other chunk
- * sizes.  A shape replaces the built-in choice only when at least 2% faster
- * (eager steps, no graph capture while tuning); the result per batch size is
+ * sizes.  A shape replaces the built-in choice only when at least 2% faster,
+ * confirmed by a second timing of both (eager steps, no graph capture while
+ * tuning); the result per batch size is
  * kept (a batch uses the entry of the nearest power of two) until the next
  * annmi_autotune or annmi_set_scan_shape.  Shapes change timing only: every
  * shape computes the same candidate keys, so answers are unchanged.  reps:
@@ -179,7 +184,9 @@ int32_t annmi_autotune(void* h, int32_t max_batch, int32_t reps, char* report, i
  * table): batches of wide_min_batch queries and more take the wide kernel
  * (0: built-in crossover), with tile configuration wide_cfg (0: built-in
  * choice; 1: 128 queries x 64 rows a wave on 4 waves; 2: 128 x 32 on 8
- * waves; 3: 64 x 64 on 8 waves) and at most chunk_rows rows per row chunk
+ * waves; 3: 64 x 64 on 8 waves; SQ8 and BF16 only, 32 x 32 MFMA tiles:
+ * 4: 128 x 32 on 8 waves, 5: 128 x 64 on 8 waves) and at most chunk_rows
+ * rows per row chunk
  * (0: built-in, else >= 2048).  Returns 0, or -1 for a configuration that does
  * not fit the LDS at this dim.  No reference counterpart (tuning, tests). */
 int32_t annmi_set_scan_shape(void* h, int32_t wide_min_batch, int32_t wide_cfg, int64_t chunk_rows);

@@ -57,3 +57,18 @@ def test_k_beyond_n_pads_through_the_raw_abi():
     np.testing.assert_array_equal(ids[:7], O.exact_topk_f64(base, q[None, :], 7)[0][0])
     assert np.all(ids[7:] == -1) and np.all(np.isinf(dd[7:]))
     L.annmi_free(h)
+
+
+def test_dimension_limits():
+    """A 16-query scan tile must fit the 160 KB of LDS: fp32 rows up to
+    d = 2544, the other modes up to the library's 4096; beyond that
+    creation fails with an error instead of a failed launch."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    for mode, d_ok, d_bad, msg in (("exact", 2544, 2545, "too large"), ("sq8", 4096, 4097, "unsupported")):
+        print(f"[case] dimension limit {mode} d={d_ok}", flush=True)
+        base = rng.standard_normal((40, d_ok), dtype=np.float32)
+        Q = rng.standard_normal((3, d_ok), dtype=np.float32)
+        ids, _ = Index(base, mode=mode, rerank=40).search(Q, 10)
+        np.testing.assert_array_equal(ids, O.exact_topk_f64(base, Q, 10)[0])
+        with pytest.raises(RuntimeError, match=msg):
+            Index(rng.standard_normal((40, d_bad), dtype=np.float32), mode=mode)
