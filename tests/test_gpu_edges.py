@@ -72,3 +72,28 @@ def test_dimension_limits():
         np.testing.assert_array_equal(ids, O.exact_topk_f64(base, Q, 10)[0])
         with pytest.raises(RuntimeError, match=msg):
             Index(rng.standard_normal((40, d_bad), dtype=np.float32), mode=mode)
+
+
+@pytest.mark.parametrize("mode", ["exact", "sq8"])
+def test_batches_beyond_max_batch_split_into_steps(mode):
+    """150 queries on a handle built for 64 run as steps of 64, 64 and 22
+    (the last shapes repeat, so steps are captured and replayed); host and
+    device buffers give the single-step answers."""
+    import torch
+
+    rng = np.random.Generator(np.random.PCG64(21))
+    base = rng.standard_normal((20000, 96), dtype=np.float32)
+    Q = rng.standard_normal((150, 96), dtype=np.float32)
+    print(f"[case] split batch {mode} B=150 max_batch=64", flush=True)
+    whole = Index(base, mode=mode, rerank=200)
+    ref_ids, ref_d = whole.search(Q, 100)
+    small = Index(base, mode=mode, rerank=200, max_batch=64)
+    for _ in range(3):
+        ids, dd = small.search(Q, 100)
+        np.testing.assert_array_equal(ids, ref_ids)
+        np.testing.assert_array_equal(dd, ref_d)
+    di, dd2 = small.search_device(torch.from_numpy(Q).to("cuda:0"), 100)
+    np.testing.assert_array_equal(di.cpu().numpy(), ref_ids)
+    np.testing.assert_array_equal(dd2.cpu().numpy(), ref_d)
+    if mode == "exact":
+        np.testing.assert_array_equal(ref_ids, O.exact_topk_f64(base, Q, 100)[0])
