@@ -1456,7 +1456,14 @@ int wide_min_batch(int kind) {
     switch (kind) {
         case SCAN_F32: return 33;
         case SCAN_FP8: return 4;
-        default: return 8;  // bf16, sq8, sq4
+        // integer codes: the wide kernel's XCD-aware row chunks stream the
+        // codes faster even for one query (10M x 384 scan time a step, one /
+        // two / four queries: SQ8 0.733 / 0.764 / 0.850 -> 0.684 / 0.694 /
+        // 0.699 ms; SQ4 0.554 / 0.595 / 0.696 -> 0.540 / 0.543 / 0.545 ms);
+        // bf16 stays on the streaming kernel below 8 (3% faster there)
+        case SCAN_SQ8:
+        case SCAN_SQ4: return 1;
+        default: return 8;  // bf16
     }
 }
 
