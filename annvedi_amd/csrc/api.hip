@@ -49,6 +49,9 @@ hipError_t launch_linear_index_query(const float* X, int64_t xstride, int64_t n,
                                      float* P, float* BT, float* R, float* bkey, uint32_t* bid, int64_t* d_ids,
                                      float* d_vals, uint32_t* d_flag, hipStream_t st);
 
+// armed timing events of the next ANNMI_LAUNCH on this thread (common.h)
+thread_local hipEvent_t t_launch_ev[2] = {nullptr, nullptr};
+
 namespace {
 
 thread_local char g_err[512] = "";
@@ -397,16 +400,40 @@ struct Profiler {
     std::vector<std::string>* scan_names;
     int ev_prep0, ev_prep1;
     std::vector<double> bytes_override, flops_override;  // per scan launch, when set
-    int ev() {
-        if (!h->prof) return -1;
+    int take() {
         if (ev_used == (int)h->ev_pool.size()) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) return -1;
             h->ev_pool.push_back(e);
         }
-        int i = ev_used++;
-        (void)hipEventRecord(h->ev_pool[i], h->st);
+        return ev_used++;
+    }
+    // an event recorded on the stream now (spans of several operations)
+    int ev() {
+        if (!h->prof) return -1;
+        const int i = take();
+        if (i >= 0) (void)hipEventRecord(h->ev_pool[i], h->st);
         return i;
+    }
+    // times the one kernel the next ANNMI_LAUNCH issues: the dispatch
+    // records the pair itself (no event packets between the kernels)
+    void arm(int& e0, int& e1) {
+        e0 = e1 = -1;
+        if (!h->prof) return;
+        e0 = take();
+        e1 = take();
+        if (e0 < 0 || e1 < 0) {
+            e0 = e1 = -1;
+            return;
+        }
+        t_launch_ev[0] = h->ev_pool[e0];
+        t_launch_ev[1] = h->ev_pool[e1];
+    }
+    // after the launch call: a pair no launch consumed (nothing launched)
+    // times nothing
+    void disarm(int& e0, int& e1) {
+        if (t_launch_ev[0]) e0 = e1 = -1;
+        t_launch_ev[0] = t_launch_ev[1] = nullptr;
     }
 };
 
@@ -431,9 +458,10 @@ bool launch_final(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
     // fp32 re-score error bound (relative, both neighbours), doubled
     const double u = 5.9604644775390625e-8;
     fa.eps_rel = 4.0 * ((h->d + 15) / 16 + 28) * u;
-    ev_f0 = pr.ev();
-    HIP_OR(launch_finalize(fa, st), false);
-    ev_f1 = pr.ev();
+    pr.arm(ev_f0, ev_f1);
+    const hipError_t le = launch_finalize(fa, st);
+    pr.disarm(ev_f0, ev_f1);
+    HIP_OR(le, false);
     return true;
 }
 
@@ -520,19 +548,22 @@ bool enqueue_flat(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
                   std::vector<double>& scan_rows, std::vector<std::string>& scan_names, int& ev_f0, int& ev_f1) {
     const bool exact = h->kind == SCAN_F32;
     hipStream_t st = h->st;
-    pr.ev_prep0 = pr.ev();
-    HIP_OR(launch_prep_queries(h->kind, h->dQ32, B, h->d, h->kdim, h->dQs, h->qn, h->qscale, h->slack, h->maxnorm,
-                               h->alpha_x, h->sq_scale, h->sq_offset, st, h->cnt, h->ovf, h->thr),
-           false);
+    pr.arm(pr.ev_prep0, pr.ev_prep1);
+    const hipError_t pe = launch_prep_queries(h->kind, h->dQ32, B, h->d, h->kdim, h->dQs, h->qn, h->qscale, h->slack,
+                                              h->maxnorm, h->alpha_x, h->sq_scale, h->sq_offset, st, h->cnt, h->ovf,
+                                              h->thr);
+    pr.disarm(pr.ev_prep0, pr.ev_prep1);
+    HIP_OR(pe, false);
     CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
-    pr.ev_prep1 = pr.ev();
     std::vector<int64_t> bounds = range_bounds(h->n, rsel, h->cap, B);
     for (size_t j = 0; j + 1 < bounds.size(); ++j) {
         if (j > 0) {
             SelectArgs sa{cb, B, rsel, h->slack, h->thr, h->ovf};
-            int e0 = pr.ev();
-            HIP_OR(launch_select_threshold(sa, st), false);
-            int e1 = pr.ev();
+            int e0, e1;
+            pr.arm(e0, e1);
+            const hipError_t le = launch_select_threshold(sa, st);
+            pr.disarm(e0, e1);
+            HIP_OR(le, false);
             sel_ev.push_back({e0, e1});
         }
         ScanArgs a{};
@@ -555,10 +586,12 @@ bool enqueue_flat(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
         a.prefilter = h->prefilter;
         a.tiled = h->tiled;
         apply_shape(shape_for(h, B), a);
-        int e0 = pr.ev();
+        int e0, e1;
         const char* vname = nullptr;
-        HIP_OR(launch_scan(a, st, &vname), false);
-        int e1 = pr.ev();
+        pr.arm(e0, e1);
+        const hipError_t le = launch_scan(a, st, &vname);
+        pr.disarm(e0, e1);
+        HIP_OR(le, false);
         scan_ev.push_back({e0, e1});
         scan_rows.push_back((double)(a.row_end - a.row_begin));
         scan_names.push_back(vname ? vname : "");
@@ -1463,17 +1496,21 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
     for (int pass = 0; pass < npass; ++pass) {
         if (pass > 0) {
             SelectArgs sa{cb, B, rsel, h->slack, h->thr, h->ovf};
-            int e0 = pr.ev();
-            HIP_OR(launch_select_threshold(sa, st), false);
-            int e1 = pr.ev();
+            int e0, e1;
+            pr.arm(e0, e1);
+            const hipError_t le = launch_select_threshold(sa, st);
+            pr.disarm(e0, e1);
+            HIP_OR(le, false);
             sel_ev.push_back({e0, e1});
         }
         ia.items = h->d_items + pstart[(size_t)pass];
         ia.nitems = pstart[(size_t)pass + 1] - pstart[(size_t)pass];
         ia.next = h->d_next + pass;
-        int e0 = pr.ev();
-        HIP_OR(launch_ivf_scan(ia, st), false);
-        int e1 = pr.ev();
+        int e0, e1;
+        pr.arm(e0, e1);
+        const hipError_t le = launch_ivf_scan(ia, st);
+        pr.disarm(e0, e1);
+        HIP_OR(le, false);
         scan_ev.push_back({e0, e1});
         scan_rows.push_back(0.0);
         scan_names.push_back("ivf_scan_kernel");

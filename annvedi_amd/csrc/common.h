@@ -1,5 +1,6 @@
 // Shared device/host definitions for the MI355X (gfx950) k-NN search library.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -10,6 +11,23 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 #define ANNMI_WAVE 64
+
+// Per-launch timing (the library's profiling): a caller arms a start/stop
+// event pair for the next ANNMI_LAUNCH on this thread, which then has the
+// dispatch itself record them (hipExtLaunchKernelGGL) instead of separate
+// event-record packets around the launch, which stalled small steps (one
+// query over 1M x 384: 0.50-0.57 ms a step timed that way, 0.44 untimed).
+// The launch consumes (clears) the pair.
+extern thread_local hipEvent_t t_launch_ev[2];
+#define ANNMI_LAUNCH(K, grid, block, lds, st, ...)                                                          \
+    do {                                                                                                    \
+        if (t_launch_ev[0]) {                                                                               \
+            hipExtLaunchKernelGGL(K, grid, block, lds, st, t_launch_ev[0], t_launch_ev[1], 0, __VA_ARGS__); \
+            t_launch_ev[0] = t_launch_ev[1] = nullptr;                                                      \
+        } else {                                                                                            \
+            hipLaunchKernelGGL(K, grid, block, lds, st, __VA_ARGS__);                                       \
+        }                                                                                                   \
+    } while (0)
 
 // Orderable unsigned image of a float: a < b (as floats, no NaN) iff
 // key(a) < key(b) (as unsigned).  Negative approximate distances (possible in

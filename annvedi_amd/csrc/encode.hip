@@ -397,6 +397,6 @@ hipError_t launch_prep_queries(int kind, const float* Q, int B, int d, int kdim,
     if (B <= 0) return hipSuccess;
     if ((cnt != nullptr) != (flags != nullptr) || (cnt != nullptr) != (thr != nullptr)) return hipErrorInvalidValue;
     QPrepArgs a{kind, Q, B, d, kdim, Qscan, qn, qscale, margin, maxnorm, alpha_x, scale, offset, cnt, flags, thr};
-    hipLaunchKernelGGL(prep_queries, dim3((unsigned)B), dim3(256), 0, st, a);
+    ANNMI_LAUNCH(prep_queries, dim3((unsigned)B), dim3(256), 0, st, a);
     return hipGetLastError();
 }

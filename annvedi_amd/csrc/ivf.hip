@@ -287,6 +287,6 @@ hipError_t launch_ivf_scan(const IvfScanArgs& a, hipStream_t st) {
     int64_t want = (int64_t)ncu * 8;
     int64_t need = (a.nitems + 3) / 4;
     unsigned grid = (unsigned)(want < need ? want : need);
-    hipLaunchKernelGGL(ivf_scan_kernel, dim3(grid), dim3(kIvfThreads), 0, st, a);
+    ANNMI_LAUNCH(ivf_scan_kernel, dim3(grid), dim3(kIvfThreads), 0, st, a);
     return hipGetLastError();
 }

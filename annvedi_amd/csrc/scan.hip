@@ -440,7 +440,7 @@ hipError_t launch_t(const ScanArgs& a, hipStream_t st) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((scan_kernel<P, QW>), dim3((unsigned)gx, (unsigned)qtiles), dim3(kThreads), lds, st, a);
+    ANNMI_LAUNCH((scan_kernel<P, QW>), dim3((unsigned)gx, (unsigned)qtiles), dim3(kThreads), lds, st, a);
     return hipGetLastError();
 }
 
@@ -994,7 +994,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
         if (v >= 0 && v < qcap) qcap = v / 64 * 64;
     }
     if (qcap < 1024) qcap = 0;
-    hipLaunchKernelGGL((scan_wide_kernel<P, QS, RS, NW, QH, QPRE>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
+    ANNMI_LAUNCH((scan_wide_kernel<P, QS, RS, NW, QH, QPRE>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
                        lds + (size_t)NW * qcap * 8 + hist_bytes, st, a, qtiles, chunk, nrc, qcap);
     return hipGetLastError();
 }
@@ -1352,7 +1352,7 @@ hipError_t launch_wide32_t(const ScanArgs& a, hipStream_t st) {
         if (v >= 0 && v < qcap) qcap = v / 64 * 64;
     }
     if (qcap < 1024) qcap = 0;
-    hipLaunchKernelGGL((scan_wide32_kernel<W, QS, RS, NW, D, QP>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
+    ANNMI_LAUNCH((scan_wide32_kernel<W, QS, RS, NW, D, QP>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
                        lds + (size_t)NW * qcap * 8 + hist_bytes, st, a, qtiles, chunk, nrc, qcap);
     return hipGetLastError();
 }

@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kThreads) void finalize(FinalArgs a) {
 
 hipError_t launch_select_threshold(const SelectArgs& a, hipStream_t st) {
     if (a.B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(select_threshold, dim3((unsigned)a.B), dim3(kThreads), kStageKeys * sizeof(uint32_t), st, a);
+    ANNMI_LAUNCH(select_threshold, dim3((unsigned)a.B), dim3(kThreads), kStageKeys * sizeof(uint32_t), st, a);
     return hipGetLastError();
 }
 
@@ -368,6 +368,6 @@ size_t finalize_smem_bytes(int d) { return sizeof(FinalSmem) + (size_t)d * sizeo
 hipError_t launch_finalize(const FinalArgs& a, hipStream_t st) {
     if (a.B <= 0) return hipSuccess;
     size_t lds = finalize_smem_bytes(a.d);
-    hipLaunchKernelGGL(finalize, dim3((unsigned)a.B), dim3(kThreads), lds, st, a);
+    ANNMI_LAUNCH(finalize, dim3((unsigned)a.B), dim3(kThreads), lds, st, a);
     return hipGetLastError();
 }
