@@ -128,7 +128,9 @@ int64_t annmi_index_bytes(void* h);
  * -1 on error. */
 int64_t annmi_export(void* h, int32_t what, void* out, int64_t bytes);
 
-/* Per-kernel timing with HIP events on the handle's own stream.  No
+/* Per-kernel timing with HIP events on the handle's own stream, recorded by
+ * the timed kernel dispatches themselves (start/stop events of
+ * hipExtLaunchKernelGGL; the IVF step's prep span with stream events).  No
  * reference counterpart (the reference's stage timers are
  * lsh_index_optimized.cpp:535-548 / experiments/cuda/gpu_filter.cu:144-162).
  * Copies the accumulated counters into out8 (when not NULL):
