@@ -25,6 +25,7 @@ def test_every_scan_shape_gives_the_same_answers(inputs, mode):
     print(f"[case] shapes mode={mode} n={N} d={D}", flush=True)
     idx = Index(base, mode=mode, rerank=200)
     ref_ids, ref_d = idx.search(Q, 100)
+    ref_small = [idx.search(Q[:b], 100) for b in (1, 3)]  # tiny batches: streaming or wide by type
     if mode == "exact":
         np.testing.assert_array_equal(ref_ids, O.exact_topk_f64(base, Q, 100)[0])
     shapes = [(1 << 30, 0, 0), (1, 0, 0), (1, 0, 4096)] + [(1, c, ch) for c in (1, 2, 3, 4, 5) for ch in (0, 8192)]
@@ -39,6 +40,10 @@ def test_every_scan_shape_gives_the_same_answers(inputs, mode):
         ids, dd = idx.search(Q, 100)
         np.testing.assert_array_equal(ids, ref_ids, err_msg=f"shape {(mb, cfg, ch)}")
         np.testing.assert_array_equal(dd, ref_d, err_msg=f"shape {(mb, cfg, ch)}")
+        for b, (ri, rd) in zip((1, 3), ref_small):
+            ids, dd = idx.search(Q[:b], 100)
+            np.testing.assert_array_equal(ids, ri, err_msg=f"shape {(mb, cfg, ch)} B={b}")
+            np.testing.assert_array_equal(dd, rd, err_msg=f"shape {(mb, cfg, ch)} B={b}")
     assert tried >= 5
 
 
