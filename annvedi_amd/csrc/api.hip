@@ -199,6 +199,10 @@ struct Handle {
     int probe_cap = 0;
     IvfItem* d_items = nullptr;
     int items_cap = 0;
+    // host-side IVF step buffers, reused across steps
+    std::vector<std::vector<uint32_t>> ivf_qlist;
+    std::vector<IvfItem> ivf_items_host;
+    std::vector<uint32_t> ivf_gq_host;
     uint32_t* d_gq = nullptr;
     int64_t gq_cap = 0;
     uint32_t* d_next = nullptr;
@@ -1391,7 +1395,13 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
     HIP_OR(hipMemcpyAsync(probe.data(), h->d_probe, probe.size() * 8, hipMemcpyDeviceToHost, st), false);
     HIP_OR(hipStreamSynchronize(st), false);
     // work items: (list chunk x group of <= 16 queries probing that list)
-    std::vector<std::vector<uint32_t>> qlist((size_t)h->nlist);
+    // per-list query lists, kept by the handle across steps (cleared, not
+    // reallocated)
+    auto& qlist = h->ivf_qlist;
+    if ((int)qlist.size() != h->nlist)
+        qlist.assign((size_t)h->nlist, {});
+    else
+        for (auto& v : qlist) v.clear();
     std::vector<int64_t> scanned((size_t)B, 0);
     for (int q = 0; q < B; ++q)
         for (int j = 0; j < nprobe; ++j) {
@@ -1407,19 +1417,48 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
     // lengths (hub lists of millions of rows included).
     int64_t S = std::max<int64_t>(256, (4 * (int64_t)rsel + nprobe - 1) / nprobe);
     S = std::max<int64_t>(16, std::min<int64_t>(S, h->cap / (2 * (int64_t)nprobe)));  // pass 0 fits the lists
-    const uint32_t CH = 256;
-    std::vector<uint32_t> gq;
-    std::vector<std::vector<IvfItem>> passes;
-    std::vector<double> pbytes, pflops;
+    const int64_t CH = 256;
     auto pass_bounds = [&](int p) -> std::pair<int64_t, int64_t> {
-        int64_t lo = p == 0 ? 0 : S, hi = S;
-        for (int i = 0; i < p; ++i) {
+        if (p == 0) return {0, S};
+        int64_t lo = S, hi = S * 8;
+        for (int i = 1; i < p; ++i) {
             lo = hi;
             hi = hi * 8;
         }
-        if (p == 0) lo = 0;
         return {lo, hi};
     };
+    // Items are counted per pass first and then written in place (one array,
+    // pass-major, each pass in (list, query group, rows) order): the host
+    // builds them while the GPU waits, ~190K items at 512 queries x nprobe 256.
+    std::vector<int64_t> pcount;
+    std::vector<double> pbytes, pflops;
+    for (int l = 0; l < h->nlist; ++l) {
+        const auto& ql = qlist[(size_t)l];
+        if (ql.empty()) continue;
+        const int64_t len = h->list_off[(size_t)l + 1] - h->list_off[(size_t)l];
+        const int64_t ngroups = ((int64_t)ql.size() + 15) / 16;
+        for (int p = 0;; ++p) {
+            auto [lo, hi] = pass_bounds(p);
+            if (lo >= len) break;
+            if ((int)pcount.size() <= p) {
+                pcount.push_back(0);
+                pbytes.push_back(0);
+                pflops.push_back(0);
+            }
+            const int64_t rows = std::min<int64_t>(len, hi) - lo;
+            pcount[(size_t)p] += ngroups * ((rows + CH - 1) / CH);
+            pbytes[(size_t)p] += (double)rows * h->xs_stride * ngroups;
+            pflops[(size_t)p] += 2.0 * rows * (double)ql.size() * h->d;
+        }
+    }
+    const int npass = (int)pcount.size();
+    std::vector<int> pstart(npass + 1, 0);
+    for (int p = 0; p < npass; ++p) pstart[p + 1] = pstart[p] + (int)pcount[(size_t)p];
+    auto& all = h->ivf_items_host;
+    auto& gq = h->ivf_gq_host;
+    all.resize((size_t)pstart[npass]);
+    gq.clear();
+    std::vector<int> pos(pstart.begin(), pstart.end() - 1);
     for (int l = 0; l < h->nlist; ++l) {
         const auto& ql = qlist[(size_t)l];
         if (ql.empty()) continue;
@@ -1427,33 +1466,16 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
         for (size_t g0 = 0; g0 < ql.size(); g0 += 16) {
             const uint32_t nq = (uint32_t)std::min<size_t>(16, ql.size() - g0);
             const uint32_t qoff = (uint32_t)gq.size();
-            for (uint32_t t = 0; t < nq; ++t) gq.push_back(ql[g0 + t]);
-            for (int p = 0;; ++p) {
+            gq.insert(gq.end(), ql.begin() + (int64_t)g0, ql.begin() + (int64_t)(g0 + nq));
+            for (int p = 0; p < npass; ++p) {
                 auto [lo, hi] = pass_bounds(p);
                 if (b0 + lo >= b1) break;
-                if ((int)passes.size() <= p) {
-                    passes.emplace_back();
-                    pbytes.push_back(0);
-                    pflops.push_back(0);
-                }
                 const int64_t e = std::min<int64_t>(b1, b0 + hi);
-                for (int64_t r = b0 + lo; r < e; r += CH) {
-                    const uint32_t re = (uint32_t)std::min<int64_t>(e, r + CH);
-                    passes[(size_t)p].push_back(IvfItem{(uint32_t)r, re, qoff, nq});
-                    pbytes[(size_t)p] += (double)(re - r) * h->xs_stride;
-                    pflops[(size_t)p] += 2.0 * (re - r) * nq * h->d;
-                }
+                for (int64_t r = b0 + lo; r < e; r += CH)
+                    all[(size_t)pos[p]++] = IvfItem{(uint32_t)r, (uint32_t)std::min<int64_t>(e, r + CH), qoff, nq};
             }
         }
     }
-    std::vector<IvfItem> all;
-    std::vector<int> pstart;
-    for (auto& v : passes) {
-        pstart.push_back((int)all.size());
-        all.insert(all.end(), v.begin(), v.end());
-    }
-    pstart.push_back((int)all.size());
-    const int npass = (int)passes.size();
     if (h->items_cap < (int)all.size()) {
         dfree(h->d_items);
         h->d_items = nullptr;
