@@ -29,6 +29,20 @@ extern thread_local hipEvent_t t_launch_ev[2];
         }                                                                                                   \
     } while (0)
 
+// Compute units of the current device, read once per process (launchers on
+// several host threads share it; a function-local static is initialised
+// exactly once).  256 on MI355X; 256 if the query fails.
+inline int device_cu_count() {
+    static const int n = [] {
+        int dev = 0, c = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+            c = 256;
+        return c;
+    }();
+    return n;
+}
+
 // Orderable unsigned image of a float: a < b (as floats, no NaN) iff
 // key(a) < key(b) (as unsigned).  Negative approximate distances (possible in
 // the ||x||^2 - 2x.q + ||q||^2 form) sort below zero.

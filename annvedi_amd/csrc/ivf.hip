@@ -276,13 +276,7 @@ hipError_t launch_centroid_update(const double* partial, const int* seg_first, c
 
 hipError_t launch_ivf_scan(const IvfScanArgs& a, hipStream_t st) {
     if (a.nitems <= 0) return hipSuccess;
-    static int ncu = 0;
-    if (ncu == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            ncu = 256;
-    }
+    const int ncu = device_cu_count();
     // persistent: a few workgroups per CU, every wave pulls items until the queue is empty
     int64_t want = (int64_t)ncu * 8;
     int64_t need = (a.nitems + 3) / 4;

@@ -414,8 +414,11 @@ thread_local const char* t_variant = "";
 
 template <class P, int QW>
 hipError_t launch_t(const ScanArgs& a, hipStream_t st) {
-    static char name[64] = "";
-    if (!name[0]) snprintf(name, sizeof(name), "scan_kernel<%s,QW=%d>", pol_name<P>(), QW);
+    static const char* const name = [] {  // built once per instantiation (thread-safe static init)
+        static char buf[64];
+        snprintf(buf, sizeof(buf), "scan_kernel<%s,QW=%d>", pol_name<P>(), QW);
+        return buf;
+    }();
     t_variant = name;
     const int qtile = 16 * QW;
     const int qtiles = (a.B + qtile - 1) / qtile;
@@ -434,11 +437,13 @@ hipError_t launch_t(const ScanArgs& a, hipStream_t st) {
     int64_t gx = want < maxb ? want : maxb;
     if (gx < 1) gx = 1;
     size_t lds = (size_t)qtile * qrow_bytes<P>(a.kdim);
-    static bool attr = false;
-    if (lds > 65536 && !attr) {  // long rows (one 16-query subtile): up to the 160 KB of LDS
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_kernel<P, QW>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
+    if (lds > 65536) {  // long rows (one 16-query subtile): up to the 160 KB of LDS
+        static const bool attr = [] {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_kernel<P, QW>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            return true;
+        }();
+        (void)attr;
     }
     ANNMI_LAUNCH((scan_kernel<P, QW>), dim3((unsigned)gx, (unsigned)qtiles), dim3(kThreads), lds, st, a);
     return hipGetLastError();
@@ -942,25 +947,22 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
 
 template <class P, int QS, int RS, int NW, int QH, bool QPRE>
 hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
-    static char name[80] = "";
-    if (!name[0])
-        snprintf(name, sizeof(name), "scan_wide_kernel<%s,QS=%d,RS=%d,NW=%d,QH=%d%s>", pol_name<P>(), QS, RS, NW, QH,
+    static const char* const name = [] {
+        static char buf[80];
+        snprintf(buf, sizeof(buf), "scan_wide_kernel<%s,QS=%d,RS=%d,NW=%d,QH=%d%s>", pol_name<P>(), QS, RS, NW, QH,
                  QPRE ? ",QP" : "");
+        return buf;
+    }();
     t_variant = name;
-    static int ncu = 0;
-    if (ncu == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            ncu = 256;
-    }
+    const int ncu = device_cu_count();
     constexpr int QT = 16 * QS * QH;
     const size_t lds = (size_t)QT * qrow_bytes<P>(a.kdim) + 5 * QT * sizeof(float) + NW * sizeof(uint32_t);
-    static bool attr = false;
-    if (!attr) {
+    static const bool attr = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide_kernel<P, QS, RS, NW, QH, QPRE>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     const int qtiles = (a.B + QT - 1) / QT;
     const int64_t rows = a.row_end - a.row_begin;
     // row chunk: a few workgroups per CU over the launch, one wave step
@@ -1297,28 +1299,25 @@ __global__ __launch_bounds__(64 * kWaves) void scan_wide32_kernel(ScanArgs a, in
 
 template <class W, int QS, int RS, int NW, int D, bool QP>
 hipError_t launch_wide32_t(const ScanArgs& a, hipStream_t st) {
-    static char name[80] = "";
-    if (!name[0])
-        snprintf(name, sizeof(name), "scan_wide32_kernel<%s,QS=%d,RS=%d,NW=%d,D=%d%s>", W::name(), QS, RS, NW, D,
+    static const char* const name = [] {
+        static char buf[80];
+        snprintf(buf, sizeof(buf), "scan_wide32_kernel<%s,QS=%d,RS=%d,NW=%d,D=%d%s>", W::name(), QS, RS, NW, D,
                  QP ? ",QP" : "");
+        return buf;
+    }();
     t_variant = name;
-    static int ncu = 0;
-    if (ncu == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            ncu = 256;
-    }
+    const int ncu = device_cu_count();
     constexpr int QT = 32 * QS;
     const int qbytes = a.kdim * W::kElemBytes;
     if (qbytes % (32 * D) != 0) return hipErrorInvalidValue;  // whole ring turns of 32-byte K steps
     const size_t lds = (size_t)QT * (qbytes + 16) + 5 * QT * sizeof(float) + NW * sizeof(uint32_t);
     if (lds + 8 * 1024 > 160 * 1024) return hipErrorInvalidValue;
-    static bool attr = false;
-    if (!attr) {
+    static const bool attr = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide32_kernel<W, QS, RS, NW, D, QP>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     const int qtiles = (a.B + QT - 1) / QT;
     const int64_t rows = a.row_end - a.row_begin;
     // row chunk as for the 16 x 16 kernel: up to 8 workgroups per CU over

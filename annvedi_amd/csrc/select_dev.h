@@ -11,7 +11,7 @@ constexpr int kBins = 2048;
 // Exclusive scan helper: returns, for the bin histogram h[kBins] in LDS, the
 // bin b where the running count first reaches `target` (1-based rank) and the
 // count of keys in bins below b.  All threads get the result.
-__device__ void find_bin(const uint32_t* h, uint32_t target, uint32_t* sh, int* out_bin, uint32_t* out_below) {
+__device__ inline void find_bin(const uint32_t* h, uint32_t target, uint32_t* sh, int* out_bin, uint32_t* out_below) {
     // each thread owns 8 consecutive bins
     const int t = threadIdx.x;
     uint32_t local = 0;
@@ -49,7 +49,7 @@ __device__ void find_bin(const uint32_t* h, uint32_t target, uint32_t* sh, int* 
 }
 
 // Key of rank `rank` (1-based) among keys[0..n), n >= rank >= 1.
-__device__ uint32_t radix_select(const float* keys, int n, uint32_t rank, uint32_t* hist, uint32_t* sh) {
+__device__ inline uint32_t radix_select(const float* keys, int n, uint32_t rank, uint32_t* hist, uint32_t* sh) {
     uint32_t prefix = 0, prefix_mask = 0;
     const int shifts[3] = {21, 10, 0};
     const int widths[3] = {11, 11, 10};
@@ -85,7 +85,7 @@ __device__ uint32_t radix_select(const float* keys, int n, uint32_t rank, uint32
 }
 
 // Key of rank `rank` (1-based) among n ordered keys already in LDS.
-__device__ uint32_t radix_select_lds(const uint32_t* keys, int n, uint32_t rank, uint32_t* hist, uint32_t* sh) {
+__device__ inline uint32_t radix_select_lds(const uint32_t* keys, int n, uint32_t rank, uint32_t* hist, uint32_t* sh) {
     uint32_t prefix = 0, prefix_mask = 0;
     const int shifts[3] = {21, 10, 0};
     const int widths[3] = {11, 11, 10};
@@ -110,7 +110,7 @@ __device__ uint32_t radix_select_lds(const uint32_t* keys, int n, uint32_t rank,
 }
 
 // Bitonic sort of n (power of two) 64-bit keys in LDS, ascending.
-__device__ void bitonic_sort(uint64_t* v, int n) {
+__device__ inline void bitonic_sort(uint64_t* v, int n) {
     for (int size = 2; size <= n; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             for (int i = threadIdx.x; i < n / 2; i += kThreads) {
