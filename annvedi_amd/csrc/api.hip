@@ -30,6 +30,7 @@
 #include "../../include/annmi.h"
 #include "common.h"
 #include "ivf.h"
+#include "robust.h"
 
 #include <numeric>
 #include <random>
@@ -175,6 +176,17 @@ struct Handle {
     float* d_dist = nullptr;
     uint32_t* scratch_keys = nullptr;
     std::vector<uint32_t> last_nrr, last_flags;
+    // exact re-run of flagged queries (robust.hip): per-query workspace for
+    // up to rb_F queries at a time
+    int rb_F = 0;
+    float* rb_keys = nullptr;
+    uint32_t* rb_hist = nullptr;
+    uint32_t* rb_sel = nullptr;
+    uint32_t* rb_cnt = nullptr;
+    uint32_t* rb_cid = nullptr;
+    int32_t* rb_rows = nullptr;
+    uint32_t* rb_flags = nullptr;
+    uint32_t* rb_nres = nullptr;
     // optional per-kernel timing with HIP events on the handle's stream
     bool prof = false;
     double prof_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // scan launches, scan ms, scan bytes, scan flops, select ms, finalize ms, prep ms, steps
@@ -317,6 +329,14 @@ void free_handle(Handle* h) {
         dfree(h->centroids);
     }
     dfree(h->scratch_keys);
+    dfree(h->rb_keys);
+    dfree(h->rb_hist);
+    dfree(h->rb_sel);
+    dfree(h->rb_cnt);
+    dfree(h->rb_cid);
+    dfree(h->rb_rows);
+    dfree(h->rb_flags);
+    dfree(h->rb_nres);
     dfree(h->li_P);
     dfree(h->li_BT);
     dfree(h->li_R);
@@ -469,8 +489,71 @@ bool launch_final(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
     return true;
 }
 
+constexpr int kRobustCap = 65536;  // rows a robust query ranks within the fp32 error window of rank k
+
+// Re-runs queries `qs` of the step just completed (queries in h->dQ32,
+// outputs rows of ids_out / d_out) with the exact all-rows path of
+// robust.hip, a bounded number at a time.  rflags / rnres receive, per
+// query, bit 0 when more than kRobustCap rows fell in the window and the
+// rows collected.
+bool run_robust(Handle* h, const std::vector<int32_t>& qs, int k, int64_t* ids_out, float* d_out,
+                std::vector<uint32_t>& rflags, std::vector<uint32_t>& rnres) {
+    hipStream_t st = h->st;
+    const int64_t n = h->n;
+    if (!h->rb_F) {
+        // d32 of every row is kept per query: at most 512 MB of it at a time
+        const int64_t fit = std::max<int64_t>(1, ((int64_t)512 << 20) / (n * 4));
+        const int F = (int)std::min<int64_t>(64, fit);
+        HIP_OR(hipMalloc(&h->rb_keys, (size_t)F * n * 4), false);
+        HIP_OR(hipMalloc(&h->rb_hist, (size_t)F * 2048 * 4), false);
+        HIP_OR(hipMalloc(&h->rb_sel, (size_t)F * 4 * 4), false);
+        HIP_OR(hipMalloc(&h->rb_cnt, (size_t)F * 4), false);
+        HIP_OR(hipMalloc(&h->rb_cid, (size_t)F * kRobustCap * 4), false);
+        HIP_OR(hipMalloc(&h->rb_rows, (size_t)F * 4), false);
+        HIP_OR(hipMalloc(&h->rb_flags, (size_t)F * 4), false);
+        HIP_OR(hipMalloc(&h->rb_nres, (size_t)F * 4), false);
+        h->rb_F = F;
+    }
+    rflags.assign(qs.size(), 0u);
+    rnres.assign(qs.size(), 0u);
+    const double u = 5.9604644775390625e-8;
+    for (size_t f0 = 0; f0 < qs.size(); f0 += (size_t)h->rb_F) {
+        const int F = (int)std::min<size_t>((size_t)h->rb_F, qs.size() - f0);
+        HIP_OR(hipMemcpyAsync(h->rb_rows, qs.data() + f0, (size_t)F * 4, hipMemcpyHostToDevice, st), false);
+        RobustArgs ra{};
+        ra.X32 = h->X32;
+        ra.x32_stride = h->kdim32;
+        ra.n = n;
+        ra.d = h->d;
+        ra.Q32 = h->dQ32;
+        ra.rows = h->rb_rows;
+        ra.F = F;
+        ra.k = k;
+        ra.keys = h->rb_keys;
+        ra.hist = h->rb_hist;
+        ra.sel = h->rb_sel;
+        ra.cnt = h->rb_cnt;
+        ra.cid = h->rb_cid;
+        ra.cap = kRobustCap;
+        ra.eps_rel = 4.0 * ((h->d + 15) / 16 + 28) * u;  // finalize's near-tie window
+        ra.out_ids = ids_out;
+        ra.out_d = d_out;
+        ra.flags = h->rb_flags;
+        ra.n_rescored = h->rb_nres;
+        HIP_OR(launch_robust(ra, st), false);
+        HIP_OR(hipMemcpyAsync(rflags.data() + f0, h->rb_flags, (size_t)F * 4, hipMemcpyDeviceToHost, st), false);
+        HIP_OR(hipMemcpyAsync(rnres.data() + f0, h->rb_nres, (size_t)F * 4, hipMemcpyDeviceToHost, st), false);
+        HIP_OR(hipStreamSynchronize(st), false);
+    }
+    return true;
+}
+
 // flag read-back, host synchronisation, event timing and accounting.
-bool complete_step(Handle* h, int B, bool exact, Profiler& pr, int ev_f0, int ev_f1) {
+// Queries the step flagged (candidate overflow, more than 2048 rows in the
+// near-tie window, an fp64 run longer than 32) are answered again by the
+// exact all-rows path (flag bit 3 marks them in annmi_last_batch_stats).
+bool complete_step(Handle* h, int B, int k, int64_t* ids_out, float* d_out, bool exact, Profiler& pr, int ev_f0,
+                   int ev_f1) {
     hipStream_t st = h->st;
     std::vector<uint32_t> flags(B), nrr(B);
     HIP_OR(hipMemcpyAsync(flags.data(), h->ovf, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
@@ -499,15 +582,15 @@ bool complete_step(Handle* h, int B, bool exact, Profiler& pr, int ev_f0, int ev
             h->prof_acc[1] += t;
             h->prof_acc[2] += by;
             h->prof_acc[3] += fl;
-            Handle::KStat& k = h->kstats[scan_names[j]];
-            k.launches += 1;
-            k.ms += t;
-            k.bytes += by;
-            k.flops += fl;
-            if (k.l_ms.size() < 4096) {
-                k.l_ms.push_back((float)t);
-                k.l_bytes.push_back((float)by);
-                k.l_flops.push_back((float)fl);
+            Handle::KStat& ks = h->kstats[scan_names[j]];
+            ks.launches += 1;
+            ks.ms += t;
+            ks.bytes += by;
+            ks.flops += fl;
+            if (ks.l_ms.size() < 4096) {
+                ks.l_ms.push_back((float)t);
+                ks.l_bytes.push_back((float)by);
+                ks.l_flops.push_back((float)fl);
             }
         }
         for (auto& e : sel_ev) h->prof_acc[4] += ms(e.first, e.second);
@@ -515,14 +598,27 @@ bool complete_step(Handle* h, int B, bool exact, Profiler& pr, int ev_f0, int ev
         h->prof_acc[6] += ms(ev_prep0, ev_prep1);
         h->prof_acc[7] += 1;
     }
-    int64_t nd = 0;
-    for (int i = 0; i < B; ++i) {
-        if (flags[i] & 3u) {
-            set_err("query %d: candidate list overflow (flags=%u); rows are not in exchangeable order", i, flags[i]);
-            return false;
+    std::vector<int32_t> redo;
+    for (int i = 0; i < B; ++i)
+        if (flags[i] & 7u) redo.push_back(i);
+    std::vector<bool> full(B, false);
+    if (!redo.empty()) {
+        std::vector<uint32_t> rflags, rnres;
+        if (!run_robust(h, redo, k, ids_out, d_out, rflags, rnres)) return false;
+        for (size_t j = 0; j < redo.size(); ++j) {
+            const int i = redo[j];
+            if (rflags[j] & 1u) {
+                set_err("query %d: more than %d rows lie within the fp32 error bound of the rank-%d distance", i,
+                        kRobustCap, k);
+                return false;
+            }
+            flags[i] |= 8u;
+            nrr[i] = rnres[j];
+            full[i] = true;
         }
-        nd += exact ? h->n : (int64_t)nrr[i];
     }
+    int64_t nd = 0;
+    for (int i = 0; i < B; ++i) nd += (exact || full[i]) ? h->n : (int64_t)nrr[i];
     h->ndist += nd;
     h->last_nrr.insert(h->last_nrr.end(), nrr.begin(), nrr.end());
     h->last_flags.insert(h->last_flags.end(), flags.begin(), flags.end());
@@ -534,7 +630,7 @@ bool finish_step(Handle* h, int B, int k, int kk, int rsel, int64_t* ids_out, fl
     (void)kk;
     int ev_f0 = -1, ev_f1 = -1;
     if (!launch_final(h, B, k, rsel, ids_out, d_out, cb, pr, ev_f0, ev_f1)) return false;
-    return complete_step(h, B, exact, pr, ev_f0, ev_f1);
+    return complete_step(h, B, k, ids_out, d_out, exact, pr, ev_f0, ev_f1);
 }
 
 
@@ -674,13 +770,13 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
             }
             if (sg->exec) {
                 HIP_OR(hipGraphLaunch(sg->exec, st), false);
-                return complete_step(h, B, exact, pr, -1, -1);
+                return complete_step(h, B, k, ids_out, d_out, exact, pr, -1, -1);
             }
         }
     }
     if (!enqueue_flat(h, B, k, rsel, ids_out, d_out, pr, scan_ev, sel_ev, scan_rows, scan_names, ev_f0, ev_f1))
         return false;
-    return complete_step(h, B, exact, pr, ev_f0, ev_f1);
+    return complete_step(h, B, k, ids_out, d_out, exact, pr, ev_f0, ev_f1);
 }
 
 bool ivf_build(Handle* h);
@@ -1402,14 +1498,25 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
         qlist.assign((size_t)h->nlist, {});
     else
         for (auto& v : qlist) v.clear();
+    // A query whose probed lists hold fewer than k rows scans every list
+    // instead (lsh_index_optimized.cpp:452-457 falls back to all points), so
+    // an answer never holds fewer than min(k, n) rows.
     std::vector<int64_t> scanned((size_t)B, 0);
-    for (int q = 0; q < B; ++q)
+    for (int q = 0; q < B; ++q) {
         for (int j = 0; j < nprobe; ++j) {
-            int64_t l = probe[(size_t)q * nprobe + j];
-            if (l < 0) continue;
-            qlist[(size_t)l].push_back((uint32_t)q);
-            scanned[(size_t)q] += h->list_off[(size_t)l + 1] - h->list_off[(size_t)l];
+            const int64_t l = probe[(size_t)q * nprobe + j];
+            if (l >= 0) scanned[(size_t)q] += h->list_off[(size_t)l + 1] - h->list_off[(size_t)l];
         }
+        if (scanned[(size_t)q] < kk) {
+            for (int l = 0; l < h->nlist; ++l) qlist[(size_t)l].push_back((uint32_t)q);
+            scanned[(size_t)q] = h->n;
+            continue;
+        }
+        for (int j = 0; j < nprobe; ++j) {
+            const int64_t l = probe[(size_t)q * nprobe + j];
+            if (l >= 0) qlist[(size_t)l].push_back((uint32_t)q);
+        }
+    }
     // geometric passes over every probed list: pass 0 takes the first S rows
     // of each list with an open filter, pass p the rows [S*8^(p-1), S*8^p);
     // before each later pass the filter becomes the rsel-th smallest key seen
@@ -1812,6 +1919,27 @@ void* annmi_load(const float* base, int64_t n, int32_t d, const annmi_params* pa
     if (hd.off[7] > bytes) {
         set_err("saved index: %lld bytes given, %lld in the index", (long long)bytes, (long long)hd.off[7]);
         return nullptr;
+    }
+    if (hd.mode == ANNMI_IVF_SQ8) {
+        // checked on the host before anything reaches the device: the list
+        // permutation becomes candidate row ids, so every entry must be a row
+        // (< n) and appear exactly once, or a kernel would read past the base
+        const uint8_t* blob = static_cast<const uint8_t*>(saved);
+        if (hd.off[5] - hd.off[4] < n * 4) {
+            set_err("saved index: list permutation truncated");
+            return nullptr;
+        }
+        std::vector<uint8_t> seen((size_t)n, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            uint32_t v;
+            memcpy(&v, blob + hd.off[4] + i * 4, 4);
+            if ((int64_t)v >= n || seen[v]) {
+                set_err("saved index: list permutation entry %lld (%u) is not a permutation of the %lld rows",
+                        (long long)i, v, (long long)n);
+                return nullptr;
+            }
+            seen[v] = 1;
+        }
     }
     p.sq_variant = hd.sq_variant;
     p.nlist = hd.nlist;

@@ -128,4 +128,92 @@ __device__ inline void bitonic_sort(uint64_t* v, int n) {
     }
 }
 
+// fp32 squared distances of NR rows in the reference order, each computed by
+// a 16-lane group; every lane of the group returns the values.  Loads are
+// issued in batches of 8 per row ahead of their FMAs (the FMA order per lane
+// is unchanged: dims l, l+16, l+32, ...).
+template <int NR>
+__device__ void l2_ref_order(const float* const (&x)[NR], const float* q, int d, int l16, float (&out)[NR]) {
+    const int d16 = d & ~15;
+    float s[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[r] = 0.f;
+    int j = l16;
+    for (; j + 16 * 7 < d16; j += 16 * 8) {
+        float xv[NR][8];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xv[r][u] = x[r][j + 16 * u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const float qv = q[j + 16 * u];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const float diff = xv[r][u] - qv;
+                s[r] = fmaf(diff, diff, s[r]);
+            }
+        }
+    }
+    for (; j < d16; j += 16) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const float diff = x[r][j] - q[j];
+            s[r] = fmaf(diff, diff, s[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        // tree: h_j = s_j + s_{j+8}; (h0+h1), (h2+h3), ...; then pairs of pairs
+        float v = s[r];
+        v += __shfl_xor(v, 8, 16);
+        v += __shfl_xor(v, 1, 16);
+        v += __shfl_xor(v, 2, 16);
+        v += __shfl_xor(v, 4, 16);
+        // tail (d % 16 dims): 8-wide block when >= 8 remain, then scalar FMAs
+        int i = d16;
+        if (d - i >= 8) {
+            // separately rounded products and sums (no contraction into FMA)
+            float p[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                float diff = x[r][i + t] - q[i + t];
+                p[t] = __fmul_rn(diff, diff);
+            }
+            float g0 = __fadd_rn(p[0], p[4]), g1 = __fadd_rn(p[1], p[5]);
+            float g2 = __fadd_rn(p[2], p[6]), g3 = __fadd_rn(p[3], p[7]);
+            v = __fadd_rn(v, __fadd_rn(__fadd_rn(g0, g2), __fadd_rn(g1, g3)));
+            i += 8;
+        }
+        for (; i < d; ++i) {
+            float diff = x[r][i] - q[i];
+            v = fmaf(diff, diff, v);
+        }
+        out[r] = v;
+    }
+}
+
+// Exact squared L2 in fp64, sequential over dims, no FMA (every term
+// (x - q)^2 of two floats is exact in fp64; only the sums round).  Loads are
+// issued 16 at a time ahead of the sums.
+__device__ inline double l2_f64(const float* x, const float* q, int d) {
+    double acc = 0.0;
+    int j = 0;
+    for (; j + 16 <= d; j += 16) {
+        float xv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) xv[u] = x[j + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            double diff = (double)xv[u] - (double)q[j + u];
+            acc = __dadd_rn(acc, __dmul_rn(diff, diff));
+        }
+    }
+    for (; j < d; ++j) {
+        double diff = (double)x[j] - (double)q[j];
+        acc = __dadd_rn(acc, __dmul_rn(diff, diff));
+    }
+    return acc;
+}
+
 }  // namespace

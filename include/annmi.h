@@ -94,7 +94,10 @@ int32_t annmi_batch_query_device(void* h, const float* dQ, int32_t B, int32_t k,
 int64_t annmi_ndist(void* h);
 
 /* Per-query diagnostics of the last batch: rows re-scored and flags
- * (bit0 candidate overflow, bit1 re-score truncated). Returns count or -1. */
+ * (bit0 candidate-list overflow, bit1 more than 2048 rows within the error
+ * bound of the rank-R key, bit2 an fp32 near-tie run at rank k longer than the
+ * fp64 window; bit3 the query was answered again by the exact all-rows path,
+ * which any of bits 0-2 triggers).  Returns count or -1. */
 int32_t annmi_last_batch_stats(void* h, uint32_t* n_rescored, uint32_t* flags, int32_t max_q);
 
 /* Release the index.  NULL-safe.  Replaces gf_free gpu_filter.cu:180-190. */

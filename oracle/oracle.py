@@ -266,11 +266,15 @@ def ivf_probe_exact(centroids, Q, nprobe):
 def ivf_search_exact(base, perm, list_off, centroids, Q, k, nprobe):
     """Exact top-k (fp64, lower id on ties) over the union of the nprobe
     nearest lists: the result an IVF list scan with a complete re-score must
-    return (lsh_index_optimized.cpp:325-333 probe, full-list candidates)."""
+    return (lsh_index_optimized.cpp:325-333 probe, full-list candidates).
+    When the probed lists hold fewer than k rows every row is a candidate
+    (lsh_index_optimized.cpp:452-457)."""
     probes = ivf_probe_exact(centroids, Q, nprobe)
     out_ids, out_d = [], []
     for qi, q in enumerate(np.atleast_2d(Q)):
         rows = np.concatenate([perm[list_off[l]:list_off[l + 1]] for l in probes[qi]]).astype(np.int64)
+        if rows.shape[0] < k:
+            rows = np.arange(base.shape[0], dtype=np.int64)
         i, dd = exact_topk_subset_f64(base, rows, q, k)
         out_ids.append(i)
         out_d.append(dd)

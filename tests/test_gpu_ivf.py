@@ -97,3 +97,26 @@ def test_ivf_through_plugin_with_reference_param_names():
     assert a.query(base[123], 1, n_probe_clusters=8)[0] == 123
     ids, dd = a.batch_query(Q, 10, n_probe_clusters=8)
     assert ids.shape == (8, 10) and np.all(np.diff(dd, axis=1) >= 0)
+
+
+def test_ivf_falls_back_to_every_row_when_probed_lists_are_short():
+    """nprobe = 1 on small lists: a query whose probed list holds fewer than
+    k rows takes every row as a candidate (lsh_index_optimized.cpp:452-457),
+    so no answer is padded with -1 while n >= k, and with every row
+    re-scored (n <= R) the answer is the exact one."""
+    print("[case] ivf fallback", flush=True)
+    base, Q = zipf_data(2000, 48, 40, 30, seed=5)
+    idx = Index(base, mode="ivf", nlist=40, kmeans_iters=6)
+    perm, off = lists_of(idx)
+    cent = idx.export("centroids")
+    probes = O.ivf_probe_exact(cent, Q, 1)[:, 0]
+    short = [int(off[l + 1] - off[l]) < 100 for l in probes]
+    assert sum(short) >= 1  # the fall-back case occurs
+    ids, dd = idx.search(Q, 100, rerank=2000, nprobe=1)
+    assert np.all(ids >= 0)
+    want_ids, want_d = O.ivf_search_exact(base, perm, off, cent, Q, 100, 1)
+    ex_ids, _ = O.exact_topk_f64(base, Q, 100)
+    for i in range(Q.shape[0]):
+        np.testing.assert_array_equal(ids[i], want_ids[i])
+        if short[i]:
+            np.testing.assert_array_equal(ids[i], ex_ids[i])

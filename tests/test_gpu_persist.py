@@ -68,3 +68,29 @@ def test_plugin_index_file_and_autotune(tmp_path):
     ib, _ = b.batch_query(Q, 100)
     np.testing.assert_array_equal(ia, ib)
     np.testing.assert_array_equal(a.query(Q[0], 100), ib[0])
+
+
+def test_corrupt_ivf_permutation_is_refused_before_upload():
+    """The list permutation of a saved IVF index becomes candidate row ids:
+    an entry past the base, or a repeated one, is refused on the host."""
+    import struct
+
+    n, d = 6000, 64
+    base = data.mixture(n, d, 16, 0.3, 85, center_seed=86)
+    a = Index(base, mode="ivf", nlist=16)
+    blob = a.save_bytes()
+    # header: magic[8] version mode n d kdim row_stride sq_variant nlist alpha reserved, then off[8]
+    off = struct.unpack_from("<8q", blob, 56)
+    perm_at = off[4]
+    assert struct.unpack_from("<I", blob, perm_at)[0] < n
+    bad = bytearray(blob)
+    struct.pack_into("<I", bad, perm_at + 4 * 17, n)  # past the last row
+    with pytest.raises(RuntimeError, match="permutation"):
+        Index(base, mode="ivf", nlist=16, saved=bytes(bad))
+    bad = bytearray(blob)
+    first = struct.unpack_from("<I", blob, perm_at)[0]
+    struct.pack_into("<I", bad, perm_at + 4 * 5, first)  # a repeated row
+    with pytest.raises(RuntimeError, match="permutation"):
+        Index(base, mode="ivf", nlist=16, saved=bytes(bad))
+    ok = Index(base, mode="ivf", nlist=16, saved=blob)  # the untouched image still loads
+    assert ok.n == n
