@@ -26,7 +26,7 @@ MODES = {"exact": ANNMI_EXACT_F32, "fp32": ANNMI_EXACT_F32, "bf16": ANNMI_BF16, 
 # every function declared in include/annmi.h
 EXPORTS = [
     "annmi_default_params", "annmi_create", "annmi_query", "annmi_batch_query", "annmi_batch_query_device",
-    "annmi_batch_query_ex", "annmi_query_linear_index",
+    "annmi_batch_query_ex", "annmi_query_linear_index", "annmi_batch_query_noalloc",
     "annmi_ndist", "annmi_last_batch_stats", "annmi_free", "annmi_clone", "annmi_last_error", "annmi_index_bytes",
     "annmi_profile", "annmi_profile_kernels", "annmi_export", "annmi_graph_count",
     "annmi_autotune", "annmi_set_scan_shape", "annmi_save", "annmi_load",
@@ -88,6 +88,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.annmi_query_linear_index.argtypes = [vp, vp, i32, i32, vp, vp]
     L.annmi_batch_query_device.restype = i32
     L.annmi_batch_query_device.argtypes = [vp, vp, i32, i32, i32, vp, vp]
+    L.annmi_batch_query_noalloc.restype = i32
+    L.annmi_batch_query_noalloc.argtypes = [vp, vp, i32, i32, i32, vp, vp]
     L.annmi_ndist.restype = i64
     L.annmi_ndist.argtypes = [vp]
     L.annmi_last_batch_stats.restype = i32

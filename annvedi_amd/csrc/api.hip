@@ -135,6 +135,7 @@ void tile_rows(const uint8_t* rows, uint8_t* tiled, int64_t n, int64_t stride, i
 
 struct Handle {
     annmi_params p{};
+    int dev = 0;  // device current at creation (its stream's device)
     int64_t n = 0;
     int d = 0;
     int kdim32 = 0;
@@ -384,7 +385,9 @@ bool ensure_ws(Handle* h, int B, int k) {
 // rsel * (rho - 1) candidates on exchangeable data.  (A longer first range
 // saves launches at small batches but costs more in the selections over its
 // candidates than it saves: measured at one query on 1M and 10M rows.)
-std::vector<int64_t> range_bounds(int64_t n, int rsel, int cap, int B) {
+// Writes the bounds (at most `maxb`: the factor is >= 4, so 2^32 rows take
+// fewer than 20) into b and returns how many; no allocation.
+int range_bounds(int64_t n, int rsel, int cap, int B, int64_t* b, int maxb) {
     // Large batches append rsel * (rho - 1) candidates per query per range
     // (each an LDS and a global atomic), small ones pay a launch and a
     // selection per range: large batches take a short first range and a
@@ -396,7 +399,8 @@ std::vector<int64_t> range_bounds(int64_t n, int rsel, int cap, int B) {
     static const int64_t env_rho = getenv("ANNMI_RANGE_RHO") ? atoll(getenv("ANNMI_RANGE_RHO")) : 0;
     static const int64_t env_b = getenv("ANNMI_RANGE_BATCH") ? atoll(getenv("ANNMI_RANGE_BATCH")) : 0;
     const bool large = B >= (env_b > 0 ? env_b : 16);
-    std::vector<int64_t> b{0};
+    int nb = 0;
+    b[nb++] = 0;
     int64_t s0 = env_s0 > 0 ? env_s0 : (large ? std::max<int64_t>(1024, 8 * (int64_t)rsel)
                                               : std::max<int64_t>(4096, 8 * (int64_t)rsel));
     s0 = (s0 + 15) / 16 * 16;  // ranges start on 16-row tiles of the tiled layout
@@ -406,12 +410,12 @@ std::vector<int64_t> range_bounds(int64_t n, int rsel, int cap, int B) {
     if (large) rho = std::min<int64_t>(rho, 4);
     if (env_rho >= 2) rho = env_rho;
     int64_t cur = std::min<int64_t>(n, s0);
-    b.push_back(cur);
-    while (cur < n) {
-        cur = std::min<int64_t>(n, cur * rho);
-        b.push_back(cur);
+    b[nb++] = cur;
+    while (cur < n && nb < maxb) {
+        cur = nb + 1 < maxb ? std::min<int64_t>(n, cur * rho) : n;
+        b[nb++] = cur;
     }
-    return b;
+    return nb;
 }
 
 // Event bookkeeping shared by the flat and IVF steps.
@@ -655,8 +659,11 @@ bool enqueue_flat(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
     pr.disarm(pr.ev_prep0, pr.ev_prep1);
     HIP_OR(pe, false);
     CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
-    std::vector<int64_t> bounds = range_bounds(h->n, rsel, h->cap, B);
-    for (size_t j = 0; j + 1 < bounds.size(); ++j) {
+    // no host allocation here unless the step is timed (the allocation-free
+    // entry point runs this with timing off)
+    int64_t bounds[64];
+    const int nb = range_bounds(h->n, rsel, h->cap, B, bounds, 64);
+    for (int j = 0; j + 1 < nb; ++j) {
         if (j > 0) {
             SelectArgs sa{cb, B, rsel, h->slack, h->thr, h->ovf};
             int e0, e1;
@@ -664,7 +671,7 @@ bool enqueue_flat(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
             const hipError_t le = launch_select_threshold(sa, st);
             pr.disarm(e0, e1);
             HIP_OR(le, false);
-            sel_ev.push_back({e0, e1});
+            if (h->prof) sel_ev.push_back({e0, e1});
         }
         ScanArgs a{};
         a.kind = h->kind;
@@ -692,9 +699,11 @@ bool enqueue_flat(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
         const hipError_t le = launch_scan(a, st, &vname);
         pr.disarm(e0, e1);
         HIP_OR(le, false);
-        scan_ev.push_back({e0, e1});
-        scan_rows.push_back((double)(a.row_end - a.row_begin));
-        scan_names.push_back(vname ? vname : "");
+        if (h->prof) {
+            scan_ev.push_back({e0, e1});
+            scan_rows.push_back((double)(a.row_end - a.row_begin));
+            scan_names.push_back(vname ? vname : "");
+        }
     }
     return launch_final(h, B, k, rsel, ids_out, d_out, cb, pr, ev_f0, ev_f1);
 }
@@ -973,6 +982,7 @@ Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params*
     }
     Handle* h = new Handle();
     h->p = p;
+    (void)hipGetDevice(&h->dev);
     h->n = n;
     h->d = d;
     h->kind = kind;
@@ -1278,6 +1288,7 @@ int32_t linear_index_impl(Handle* h, const float* Q, int32_t B, int32_t k, int64
 Handle* clone_impl(Handle* src) {
     Handle* h = new Handle();
     h->p = src->p;
+    h->dev = src->dev;
     h->n = src->n;
     h->d = src->d;
     h->kdim32 = src->kdim32;
@@ -1758,6 +1769,162 @@ bool autotune_impl(Handle* h, int max_batch, int reps) {
     return true;
 }
 
+// ------------------------------------------------ allocation-free queries --
+// annmi_batch_query_noalloc runs a flat step on a static device array instead
+// of the handle's workspace: up to kNoAllocMaxBatch queries, candidate lists
+// of kNoAllocCap keys.  Sections (256-byte aligned): fp32 queries, scan
+// operands, seven per-query words, candidate keys, candidate ids.
+constexpr int kNoAllocMaxBatch = 256;
+constexpr int kNoAllocCap = 16384;
+constexpr size_t na_sec(size_t bytes) { return (bytes + 255) / 256 * 256; }
+constexpr size_t kNaQ32 = na_sec((size_t)kNoAllocMaxBatch * kMaxDim * 4);
+constexpr size_t kNaQs = na_sec((size_t)kNoAllocMaxBatch * kMaxDim * 4);
+constexpr size_t kNaWord = na_sec((size_t)kNoAllocMaxBatch * 4);
+constexpr size_t kNaList = na_sec((size_t)kNoAllocMaxBatch * kNoAllocCap * 4);
+constexpr size_t kScratchBytes = kNaQ32 + kNaQs + 7 * kNaWord + 2 * kNaList;  // 40 MiB + 7 KiB
+
+std::mutex g_noalloc_mu;         // one call at a time per process uses the array
+void* g_scratch_dev[64] = {};    // the array's address on each device (hipGetSymbolAddress)
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+// Address of `scratch` on device `dev`, looked up once per device with that
+// device current (the caller holds g_noalloc_mu and has made it current).
+void* scratch_on(int dev) {
+    if (dev < 0 || dev >= 64) {
+        set_err("device %d out of range for the allocation-free entry point", dev);
+        return nullptr;
+    }
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_err("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    return g_scratch_dev[dev];
+}
+
+// The handle's workspace pointers, swapped for sections of the scratch array
+// for one call and restored on scope exit.
+struct WorkspaceSwap {
+    Handle* h;
+    float *dQ32, *qn, *qscale, *slack, *thr, *ckey;
+    void* dQs;
+    uint32_t *cnt, *crid, *ovf, *nrr;
+    int cap;
+    bool prof;
+    WorkspaceSwap(Handle* hh, uint8_t* base) : h(hh) {
+        dQ32 = h->dQ32, qn = h->qn, qscale = h->qscale, slack = h->slack, thr = h->thr, ckey = h->ckey;
+        dQs = h->dQs, cnt = h->cnt, crid = h->crid, ovf = h->ovf, nrr = h->nrr, cap = h->cap, prof = h->prof;
+        uint8_t* at = base;
+        auto take = [&](size_t bytes) {
+            uint8_t* p = at;
+            at += bytes;
+            return p;
+        };
+        h->dQ32 = reinterpret_cast<float*>(take(kNaQ32));
+        h->dQs = take(kNaQs);
+        h->qn = reinterpret_cast<float*>(take(kNaWord));
+        h->qscale = reinterpret_cast<float*>(take(kNaWord));
+        h->slack = reinterpret_cast<float*>(take(kNaWord));
+        h->thr = reinterpret_cast<float*>(take(kNaWord));
+        h->cnt = reinterpret_cast<uint32_t*>(take(kNaWord));
+        h->ovf = reinterpret_cast<uint32_t*>(take(kNaWord));
+        h->nrr = reinterpret_cast<uint32_t*>(take(kNaWord));
+        h->ckey = reinterpret_cast<float*>(take(kNaList));
+        h->crid = reinterpret_cast<uint32_t*>(take(kNaList));
+        h->cap = kNoAllocCap;
+        h->prof = false;  // event timing would create events
+    }
+    ~WorkspaceSwap() {
+        h->dQ32 = dQ32, h->qn = qn, h->qscale = qscale, h->slack = slack, h->thr = thr, h->ckey = ckey;
+        h->dQs = dQs, h->cnt = cnt, h->crid = crid, h->ovf = ovf, h->nrr = nrr, h->cap = cap, h->prof = prof;
+    }
+};
+
+int32_t noalloc_impl(Handle* h, const float* dQ, int32_t B, int32_t k, int32_t R, int64_t* ids, float* dists) {
+    g_err[0] = 0;
+    if (!h) {
+        set_err("null handle");
+        return -1;
+    }
+    if (h->ivf) {
+        set_err("the allocation-free entry point serves the flat modes (exact, bf16, fp8, sq8, sq4), not IVF");
+        return -1;
+    }
+    if (B < 0 || k <= 0 || (B > 0 && (!dQ || !ids || !dists))) {
+        set_err("bad arguments: B=%d k=%d", B, k);
+        return -1;
+    }
+    if (B > kNoAllocMaxBatch) {
+        set_err("B=%d exceeds the %d queries the allocation-free entry point's scratch holds", B, kNoAllocMaxBatch);
+        return -1;
+    }
+    if (k > 2048) {
+        set_err("k=%d exceeds 2048", k);
+        return -1;
+    }
+    const bool exact = h->kind == SCAN_F32;
+    const int kk = (int)std::min<int64_t>(k, h->n);
+    if (R <= 0) R = h->p.rerank > 0 ? h->p.rerank : 4 * k;
+    int rsel = exact ? kk : std::max(kk, R);
+    rsel = (int)std::min<int64_t>(rsel, h->n);
+    if (!exact && rsel > 2048) {
+        set_err("rerank R=%d exceeds the 2048-row re-score limit", rsel);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    std::lock_guard<std::mutex> lk2(g_noalloc_mu);
+    int cur = 0;
+    HIP_OR(hipGetDevice(&cur), -1);
+    if (cur != h->dev) HIP_OR(hipSetDevice(h->dev), -1);
+    struct DeviceBack {
+        int cur, dev;
+        ~DeviceBack() {
+            if (cur != dev) (void)hipSetDevice(cur);
+        }
+    } back{cur, h->dev};
+    uint8_t* base = static_cast<uint8_t*>(scratch_on(h->dev));
+    if (!base) return -1;
+    h->last_nrr.clear();
+    h->last_flags.clear();
+    if (B == 0) return kk;
+    uint32_t flags[kNoAllocMaxBatch], nrr[kNoAllocMaxBatch];
+    {
+        WorkspaceSwap ws(h, base);
+        HIP_OR(hipMemcpyAsync(h->dQ32, dQ, (size_t)B * h->d * 4, hipMemcpyDeviceToDevice, h->st), -1);
+        std::vector<std::pair<int, int>> scan_ev, sel_ev;  // stay empty: nothing is timed
+        std::vector<double> scan_rows;
+        std::vector<std::string> scan_names;
+        int ev_used = 0, ev_f0 = -1, ev_f1 = -1;
+        Profiler pr{h, ev_used, &scan_ev, &sel_ev, &scan_rows, &scan_names, -1, -1};
+        if (!enqueue_flat(h, B, k, rsel, ids, dists, pr, scan_ev, sel_ev, scan_rows, scan_names, ev_f0, ev_f1))
+            return -1;
+        HIP_OR(hipMemcpyAsync(flags, h->ovf, (size_t)B * 4, hipMemcpyDeviceToHost, h->st), -1);
+        HIP_OR(hipMemcpyAsync(nrr, h->nrr, (size_t)B * 4, hipMemcpyDeviceToHost, h->st), -1);
+        HIP_OR(hipStreamSynchronize(h->st), -1);
+    }
+    int64_t nd = 0;
+    for (int i = 0; i < B; ++i) {
+        if (flags[i] & 7u) {
+            set_err("query %d needs the exact all-rows path (flags %u), which allocates workspace: answer it with "
+                    "annmi_batch_query_device",
+                    i, flags[i]);
+            return -1;
+        }
+        nd += exact ? h->n : (int64_t)nrr[i];
+    }
+    h->ndist += nd;
+    return kk;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1797,6 +1964,11 @@ int32_t annmi_batch_query_ex(void* h, const float* Q, int32_t B, int32_t k, int3
 int32_t annmi_batch_query_device(void* h, const float* dQ, int32_t B, int32_t k, int32_t rerank, int64_t* d_ids,
                                  float* d_dists) {
     return batch_impl(static_cast<Handle*>(h), dQ, B, k, rerank, d_ids, d_dists, true, 0);
+}
+
+int32_t annmi_batch_query_noalloc(void* h, const float* dQ, int32_t B, int32_t k, int32_t rerank, int64_t* d_ids,
+                                  float* d_dists) {
+    return noalloc_impl(static_cast<Handle*>(h), dQ, B, k, rerank, d_ids, d_dists);
 }
 
 int64_t annmi_ndist(void* h) { return h ? static_cast<Handle*>(h)->ndist.load() : 0; }

@@ -88,6 +88,23 @@ int32_t annmi_query_linear_index(void* h, const float* Q, int32_t B, int32_t k, 
 int32_t annmi_batch_query_device(void* h, const float* dQ, int32_t B, int32_t k, int32_t rerank, int64_t* d_ids,
                                  float* d_dists);
 
+/* Same as annmi_batch_query_device (device pointers, synchronous) but
+ * allocates nothing: no hipMalloc / hipFree and no allocating library
+ * routine, on the device or the host.  No counterpart in the reference: it is
+ * for callers that cannot allocate once serving has started (tight serving
+ * loops, harnesses that forbid allocation after set-up).  The step's
+ * workspace is a static device array of 40 MiB held by every device the
+ * library runs on until the process ends; each call uses it whole, so calls
+ * are serialised across the process (calls that overlapped on one device
+ * would race on it).  Limits: flat modes only (exact, bf16, fp8, sq8, sq4;
+ * not IVF), B <= 256 queries per call, k <= 2048, candidate lists of 16384
+ * keys per query.  A query the regular step flags (annmi_last_batch_stats
+ * bits 0-2) would need the exact all-rows path, which allocates: the call
+ * then returns -1 with an error naming the query.  Returns min(k, n) or -1.
+ * annmi_last_batch_stats reports nothing for these calls. */
+int32_t annmi_batch_query_noalloc(void* h, const float* dQ, int32_t B, int32_t k, int32_t rerank, int64_t* d_ids,
+                                  float* d_dists);
+
 /* Full-precision distance evaluations so far.
  * Replaces: LinearIndex::total_distances_count  src/linear_index.cpp:123 (N per query)
  *           gf_ndist                             gpu_filter.cu:178 (R per query) */
