@@ -26,6 +26,8 @@
 //   orc_sq4_train/encode     submission/src/hnsw.cpp:147-183 (nibble layout)
 //   orc_sq4_l2               submission/src/simd.h:292-318 (AVX-512 branch, no threshold)
 //   orc_f32_direct_l2        gpu_filter.cu:161-170 rerank loop (scalar acc += v*v)
+//   orc_kmeans_init          lsh_index_optimized.cpp:553-561 (initial centroid rows)
+//   orc_ivf_kmeans           lsh_index_optimized.cpp:551-616, :32-73 (k-means, one thread)
 //
 // Parity pins for each function are listed in the header of oracle/oracle.py.
 #include <algorithm>
@@ -395,6 +397,58 @@ void orc_kmeans_init(int64_t n, int nlist, uint32_t seed, int64_t* init_rows, in
     for (int c = 0; c < nlist; ++c) init_rows[c] = r[(size_t)(c % n)];
     if (rand_indices)
         for (int64_t i = 0; i < n; ++i) rand_indices[i] = r[(size_t)i];
+}
+
+// competitors/ivf_lsh/src/lsh_index_optimized.cpp:551-616 (run_kmeans) with
+// the workers :32-73, one thread: initial centroids from orc_kmeans_init;
+// each round assigns every row to the argmin of compute_l2_distance
+// (src/simd.h:32-64 order, strict <: the lowest centroid wins a tie), sums
+// the rows of each cluster in float in row order, and sets the centroid to
+// sum / count, or to row rand_indices[(c * 37) % n] when the cluster is
+// empty.  The reference sums per thread and then over threads, so its float
+// sums match these only where they are exact (e.g. integer coordinates with
+// sums below 2^24); its assignment matches wherever the centroids do.
+// n_clusters is clamped to n (:140).  centroids: [nlist x dim], assign: [n].
+void orc_ivf_kmeans(const float* data, int64_t n, int64_t dim, int nlist, int iters, uint32_t seed,
+                    float* centroids, int32_t* assign) {
+    if (nlist > n) nlist = (int)n;
+    std::vector<int64_t> init((size_t)nlist), ri((size_t)n);
+    orc_kmeans_init(n, nlist, seed, init.data(), ri.data());
+    for (int c = 0; c < nlist; ++c)
+        std::memcpy(centroids + (int64_t)c * dim, data + init[(size_t)c] * dim, (size_t)dim * sizeof(float));
+    std::vector<float> sums((size_t)nlist * dim);
+    std::vector<int64_t> counts((size_t)nlist);
+    for (int it = 0; it < iters; ++it) {
+        for (int64_t i = 0; i < n; ++i) {
+            float best = std::numeric_limits<float>::max();
+            int bc = 0;
+            for (int c = 0; c < nlist; ++c) {
+                const float dd = orc_l2_linear_index(data + i * dim, centroids + (int64_t)c * dim, dim,
+                                                     std::numeric_limits<float>::max());
+                if (dd < best) {
+                    best = dd;
+                    bc = c;
+                }
+            }
+            assign[i] = bc;
+        }
+        std::fill(sums.begin(), sums.end(), 0.0f);
+        std::fill(counts.begin(), counts.end(), 0);
+        for (int64_t i = 0; i < n; ++i) {
+            const int c = assign[i];
+            counts[(size_t)c]++;
+            for (int64_t d = 0; d < dim; ++d) sums[(size_t)(c * dim + d)] += data[i * dim + d];
+        }
+        for (int c = 0; c < nlist; ++c) {
+            if (counts[(size_t)c] > 0) {
+                for (int64_t d = 0; d < dim; ++d)
+                    centroids[(int64_t)c * dim + d] = sums[(size_t)(c * dim + d)] / (float)counts[(size_t)c];
+            } else {
+                const int64_t r = ri[(size_t)((c * 37) % n)];
+                std::memcpy(centroids + (int64_t)c * dim, data + r * dim, (size_t)dim * sizeof(float));
+            }
+        }
+    }
 }
 
 }  // extern "C"

@@ -30,6 +30,17 @@ Parity pins (each a check a plausible mistake would fail):
   float64 dequantised distance.
 * ``recall`` (orthogonal-competition/evaluator.py:149-157): hand-computed
   cases including ties at the threshold.
+* ``kmeans_init``, ``ivf_kmeans``, ``ivf_reference_query``
+  (lsh_index_optimized.cpp:304-516, 551-616): the compiled reference's IVF
+  answers on integer-coordinate data, where every float sum and distance is
+  exact and so the whole chain -- the mt19937 shuffle's initial rows, 15
+  rounds, the probe order, the fall-back to all rows, the fp32 ranking -- must
+  match bit for bit (tests/golden/ivf_ref.npz); and on separated clusters
+  seeded one per cluster, where only the partition must match.
+* ``exact_topk_subset_f64``: brute force over the subset in numpy float64.
+* ``f32_direct_l2`` (gpu_filter.cu:161-170): integer-valued rows (every sum
+  exact), a sum that only the sequential order absorbs, a product that FMA
+  contraction would not round.
 """
 from __future__ import annotations
 
@@ -95,6 +106,9 @@ def lib():
         L.orc_sq4_l2.argtypes = [_u8p, _f32p, _f32p, _i64]
         L.orc_kmeans_init.restype = None
         L.orc_kmeans_init.argtypes = [_i64, _i32, ctypes.c_uint32, _i64p, _i64p]
+        L.orc_ivf_kmeans.restype = None
+        L.orc_ivf_kmeans.argtypes = [_f32p, _i64, _i64, _i32, _i32, ctypes.c_uint32, _f32p,
+                                     np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
         L.orc_sq4_nibble.restype = _i32
         L.orc_sq4_nibble.argtypes = [_u8p, _i64]
         _lib = L
@@ -198,20 +212,6 @@ def sq8_l2_f64(code_row, q_shifted, scale) -> float:
     return float(lib().orc_sq8_l2_f64(c, _f32(q_shifted), _f32(scale), c.shape[0]))
 
 
-def sq8_topk_f64(codes, scale, offset, Q, r):
-    """Top-r rows by float64 dequantised SQ8 distance, ties by lower id."""
-    codes = np.ascontiguousarray(codes, dtype=np.uint8)
-    Q = _f32(np.atleast_2d(Q))
-    deq = codes.astype(np.float64) * scale.astype(np.float64)[None, :]
-    out = []
-    for q in Q:
-        qs = (q.astype(np.float32) - offset).astype(np.float64)
-        dd = ((deq - qs[None, :]) ** 2).sum(axis=1)
-        order = np.lexsort((np.arange(dd.shape[0]), dd))[:r]
-        out.append(order)
-    return np.stack(out)
-
-
 # ------------------------------------------------------------------ SQ4 ----
 def sq4_train(x):
     x = _f32(x)
@@ -255,6 +255,58 @@ def kmeans_init(n, nlist, seed=1337):
     ri = np.empty(n, dtype=np.int64)
     lib().orc_kmeans_init(int(n), int(nlist), int(seed), init, ri)
     return init, ri
+
+
+def ivf_kmeans(x, nlist, iters=15, seed=1337):
+    """(centroids float32 [nlist', d], assign int32 [n]) of the reference's
+    k-means (lsh_index_optimized.cpp:551-616) run on one thread; nlist' =
+    min(nlist, n)."""
+    x = _f32(x)
+    n, d = x.shape
+    m = min(int(nlist), n)
+    cent = np.empty((m, d), dtype=np.float32)
+    assign = np.empty(n, dtype=np.int32)
+    lib().orc_ivf_kmeans(x, n, d, m, int(iters), int(seed), cent, assign)
+    return cent, assign
+
+
+def ivf_lists(assign, nlist):
+    """(perm, list offsets): rows grouped by cluster, ascending id inside a
+    cluster (the reordered layout of lsh_index_optimized.cpp:155-173)."""
+    assign = np.asarray(assign, dtype=np.int64)
+    perm = np.argsort(assign, kind="stable").astype(np.int64)
+    off = np.zeros(nlist + 1, dtype=np.int64)
+    np.cumsum(np.bincount(assign, minlength=nlist), out=off[1:])
+    return perm, off
+
+
+def ivf_reference_query(base, centroids, perm, list_off, Q, k, nprobe):
+    """lsh_index_optimized.cpp:304-516 with n_tables 1, n_bits 0 (one bucket
+    per list) and refine_r at least the candidate count, so that every
+    candidate is refined:
+      probe  the nprobe smallest (fp32 compute_l2_distance(q, centroid), list)
+             pairs (:325-333);
+      gather the probed lists' rows; every row when they hold fewer than k
+             (:452-457);
+      rank   candidates by fp32 compute_l2_distance(row, q) (:485-496).
+    The reference's last two sorts compare distances only, so the order of
+    exactly equal fp32 distances is unpinned; here they go by id.  Returns
+    int64 ids [nq, k], padded with -1 past the candidates (:506-511)."""
+    base = _f32(base)
+    centroids = _f32(centroids)
+    Q = _f32(np.atleast_2d(Q))
+    n = base.shape[0]
+    out = np.full((Q.shape[0], k), -1, dtype=np.int64)
+    for qi, q in enumerate(Q):
+        cd = np.array([l2_linear_index(q, c) for c in centroids], dtype=np.float32)
+        probes = np.lexsort((np.arange(len(cd)), cd))[:min(nprobe, len(cd))]
+        rows = np.concatenate([perm[list_off[l]:list_off[l + 1]] for l in probes]).astype(np.int64)
+        if rows.shape[0] < k:
+            rows = np.arange(n, dtype=np.int64)
+        dd = np.array([l2_linear_index(base[r], q) for r in rows], dtype=np.float32)
+        order = np.lexsort((rows, dd))[:k]
+        out[qi, :len(order)] = rows[order]
+    return out
 
 
 def ivf_probe_exact(centroids, Q, nprobe):

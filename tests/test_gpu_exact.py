@@ -89,25 +89,49 @@ def test_device_tensors_and_borrowed_base():
     check_exact(base, Q, 100, ids.cpu().numpy(), dd.cpu().numpy())
 
 
-def test_exact_one_million_rows():
-    """Full C2 base size (1M x 384); oracle on a few queries."""
+@pytest.fixture(scope="module")
+def c2_base():
+    """The 1M x 384 plain-fp32 workload (1024 clusters, sigma 0.3, unit
+    rows), built in HBM, and its host copy for the oracle."""
     import torch
 
     base = data.mixture_torch(1_000_000, 384, 1024, 0.3, 2001, center_seed=1001)
-    Q = data.mixture_torch(64, 384, 1024, 0.3, 3001, center_seed=1001)
+    yield base, base.cpu().numpy()
+    del base
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("B", [1, 64, 1024, 10000])
+def test_exact_one_million_rows(c2_base, B):
+    """Full C2 base size (1M x 384) at the C2 batch sizes (10000 queries run
+    as three steps of the 4096-query workspace); oracle on a sample of each
+    batch, size-independent properties on every query."""
+    import torch
+
+    print(f"[case] exact 1M x 384 B={B}", flush=True)
+    base, hb = c2_base
+    Q = data.mixture_torch(B, 384, 1024, 0.3, 3001 + B, center_seed=1001)
     idx = Index(base, mode="exact")
     ids, dd = idx.search_device(Q, 100)
     ids, dd = ids.cpu().numpy(), dd.cpu().numpy()
-    hb = base.cpu().numpy()
     hq = Q.cpu().numpy()
-    sel = [0, 31, 63]
+    sel = sorted({0, B // 2, B - 1, min(B - 1, 4095), min(B - 1, 4096), min(B - 1, 8193)})
     check_exact(hb, hq[sel], 100, ids[sel], dd[sel])
-    # size-independent properties on every query: sorted by (d, id), unique ids
-    assert np.all(np.diff(dd, axis=1) >= 0)
+    # size-independent properties on every query: ranked by (fp64, id) with
+    # the reference's fp32 values reported, so a reported value may step
+    # down by a rounding error only where the fp64 order says so
+    steps = np.argwhere(np.diff(dd, axis=1) < 0)
+    print(f"fp32 values stepping down inside a near-tie run: {len(steps)}")
+    for i, j in steps:
+        a, b = O.l2_f64(hb[ids[i, j]], hq[i]), O.l2_f64(hb[ids[i, j + 1]], hq[i])
+        assert a < b or (a == b and ids[i, j] < ids[i, j + 1]), (i, j)
+        assert dd[i, j] - dd[i, j + 1] <= 1e-6 * dd[i, j]
+    assert np.all((ids >= 0) & (ids < 1_000_000))
     assert all(len(set(r)) == 100 for r in ids)
-    # self query: a base row is its own nearest neighbour at distance 0
-    ids2, dd2 = idx.search_device(base[[5, 999_999]], 10)
-    assert ids2[:, 0].tolist() == [5, 999_999] and float(dd2[:, 0].abs().max()) == 0.0
+    if B == 64:
+        # self query: a base row is its own nearest neighbour at distance 0
+        ids2, dd2 = idx.search_device(base[[5, 999_999]], 10)
+        assert ids2[:, 0].tolist() == [5, 999_999] and float(dd2[:, 0].abs().max()) == 0.0
     del idx
     torch.cuda.empty_cache()
 

@@ -17,8 +17,8 @@ def ds():
     return base, Q, ref_ids, ref_d
 
 
-@pytest.mark.parametrize("mode,R,min_recall", [("bf16", 400, 0.99), ("fp8", 1000, 0.97),
-                                               ("sq8", 1000, 0.99), ("sq4", 1000, 0.90)])
+@pytest.mark.parametrize("mode,R,min_recall", [("bf16", 400, 0.99), ("fp8", 1000, 0.99),
+                                               ("sq8", 1000, 0.99), ("sq4", 1000, 0.99)])
 def test_quantized_recall_and_rescore(ds, mode, R, min_recall):
     base, Q, ref_ids, ref_d = ds
     print(f"[case] {mode} R={R}", flush=True)
@@ -68,18 +68,21 @@ def test_gpu_filter_abi_drop_in():
 
 @pytest.mark.parametrize("mode", ["bf16", "fp8", "sq8", "sq4"])
 def test_large_batch_quantized_consistent(mode):
-    """Query-stationary kernel (B = 256) vs streaming kernel (B = 1): with
-    R = 600 both re-score supersets of the true top-100 on this data, so the
-    answers agree with each other and with the oracle."""
+    """Query-stationary kernel (B = 256) vs streaming kernel (B = 1): a
+    query's answer does not depend on the batch it is in (same keys, same
+    per-query thresholds), and with R = 600 the re-scored rows hold the true
+    top-100 on this data."""
+    print(f"[case] batch invariance {mode}", flush=True)
     base, Q = data.workload("exact-1m-384", n=120_000, n_queries=256)
     idx = Index(base, mode=mode, rerank=600)
     ids, dd = idx.search(Q, 100)
     ref_ids, _ = O.exact_topk_f64(base, Q[:8], 100)
-    rec = np.mean([len(set(ids[i]) & set(ref_ids[i])) / 100 for i in range(8)])
-    assert rec >= 0.97, rec
-    for i in (0, 200):
+    rec = O.recall_at_k(base, Q[:8], ids[:8], ref_ids, 100).mean()
+    assert rec == 1.0, rec
+    for i in (0, 200, 255):
         i1, d1 = idx.search(Q[i:i + 1], 100)
-        assert len(set(i1[0]) & set(ids[i])) >= 97
+        np.testing.assert_array_equal(i1[0], ids[i])
+        np.testing.assert_array_equal(d1[0], dd[i])
 
 
 @pytest.mark.parametrize("mode", ["bf16", "fp8", "sq8", "sq4"])

@@ -166,3 +166,87 @@ def test_recall_hand_cases():
     assert O.recall(true_d, np.array([[0.1, 0.3, 0.31]]), 3)[0] == pytest.approx(2 / 3)
     # ties at the threshold count as hits
     assert O.recall(true_d, np.array([[0.3, 0.3, 0.3]]), 3)[0] == 1.0
+
+
+# ---------------------------------------------------------------- IVF ------
+IVF_GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "ivf_ref.npz"))
+
+
+@pytest.mark.parametrize("case", ["line", "clusters"])
+def test_ivf_restatement_matches_compiled_reference(case):
+    """k-means (initial rows of shuffle(iota(n), mt19937(1337)), 15 rounds,
+    empty-cluster rule) and the query (probe order, fall-back to all rows
+    when the probed lists hold fewer than k rows, fp32 ranking) against the
+    reference's answers (lsh_index_optimized.cpp:304-516, 551-616).  On the
+    integer-coordinate case every sum is exact, so this pins the initial
+    rows: a different seed, shuffle or first-round rule changes the lists."""
+    from tests.golden.make_ivf_golden import CASES, K, NPROBES
+
+    base, Q, nlist = CASES[case]()
+    assert checksum(base, Q) == str(IVF_GOLD[f"{case}__checksum"]), "input generator drifted"
+    want = IVF_GOLD[f"{case}__ids"]
+    cent, assign = O.ivf_kmeans(base, nlist)
+    perm, off = O.ivf_lists(assign, nlist)
+    for j, p in enumerate(NPROBES):
+        got = O.ivf_reference_query(base, cent, perm, off, Q, K, p)
+        np.testing.assert_array_equal(got, want[:, j], err_msg=f"nprobe {p}")
+
+
+def test_ivf_line_case_pins_the_initial_rows():
+    """A plausible slip in the initialisation (another seed, rows taken in
+    iota order) changes the reference's answers on the exact case."""
+    from tests.golden.make_ivf_golden import CASES, K
+
+    base, Q, nlist = CASES["line"]()
+    want = IVF_GOLD["line__ids"][:, 2]  # nprobe 4: mostly no fall-back
+    for seed in (1338, 42):
+        cent, assign = O.ivf_kmeans(base, nlist, seed=seed)
+        perm, off = O.ivf_lists(assign, nlist)
+        assert not np.array_equal(O.ivf_reference_query(base, cent, perm, off, Q, K, 4), want)
+    init, _ = O.kmeans_init(base.shape[0], nlist)
+    assert len(set(init.tolist())) == nlist and not np.array_equal(init, np.arange(nlist))
+
+
+def test_ivf_fall_back_rule_matters_on_the_golden_case():
+    """Without the fall-back to all rows the probed lists of nprobe 1 hold
+    fewer than k rows and the answer would be padded with -1; the reference
+    answers every query with k rows."""
+    want = IVF_GOLD["clusters__ids"]
+    assert np.all(want >= 0)
+
+
+def test_exact_topk_subset_brute_force():
+    rng = np.random.Generator(np.random.PCG64(11))
+    base = rng.standard_normal((300, 24), dtype=np.float32)
+    base[50] = base[7]  # a duplicate: tie broken by the lower id
+    q = rng.standard_normal(24, dtype=np.float32)
+    cand = np.sort(rng.choice(300, size=120, replace=False))
+    cand = np.union1d(cand, [7, 50]).astype(np.int64)
+    ids, dd = O.exact_topk_subset_f64(base, cand, q, 60)
+    d64 = ((base[cand].astype(np.float64) - q.astype(np.float64)) ** 2).sum(1)
+    order = np.lexsort((cand, d64))[:60]
+    np.testing.assert_array_equal(ids, cand[order])
+    np.testing.assert_allclose(dd, d64[order], rtol=1e-12)
+    # k beyond the subset clamps
+    ids2, _ = O.exact_topk_subset_f64(base, cand[:5], q, 60)
+    assert ids2.shape == (5,)
+
+
+def test_f32_direct_l2_closed_forms():
+    """gpu_filter.cu:161-170: acc += v*v in float, in dim order, product
+    rounded on its own."""
+    z = np.zeros(64, dtype=np.float32)
+    # integer-valued rows: every product and partial sum is exact
+    a = np.arange(64, dtype=np.float32) - 20
+    assert O.f32_direct_l2(a, z) == np.float32(((a.astype(np.int64)) ** 2).sum())
+    # sequential order: 2^24 first absorbs every later +1 (a tree would not)
+    a = np.ones(64, dtype=np.float32)
+    a[0] = 4096.0
+    assert O.f32_direct_l2(a, z) == np.float32(2.0 ** 24)
+    # separately rounded product: 1 + (1 + 3*2^-13)^2 is 2 + 3*2^-12 after a
+    # rounded product and a tie to even; a fused multiply-add keeps the
+    # product's low bits and rounds up by one ulp of 2
+    v = np.float32(1.0 + 3.0 * 2.0 ** -13)
+    got = O.f32_direct_l2(np.array([1.0, v], np.float32), np.zeros(2, np.float32))
+    assert got == np.float32(2.0 + 3.0 * 2.0 ** -12)
+    assert got != np.float32(2.0 + 3.0 * 2.0 ** -12 + 2.0 ** -22)
