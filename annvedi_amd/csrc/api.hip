@@ -182,7 +182,9 @@ struct Handle {
     int rb_F = 0;
     float* rb_keys = nullptr;
     uint32_t* rb_hist = nullptr;
-    uint32_t* rb_sel = nullptr;
+    uint64_t* rb_sel = nullptr;
+    uint32_t* gf_cid = nullptr;  // gf_query's collected rows when r exceeds the shared list
+    int64_t gf_cap = 0;
     uint32_t* rb_cnt = nullptr;
     uint32_t* rb_cid = nullptr;
     int32_t* rb_rows = nullptr;
@@ -338,6 +340,7 @@ void free_handle(Handle* h) {
     dfree(h->rb_rows);
     dfree(h->rb_flags);
     dfree(h->rb_nres);
+    dfree(h->gf_cid);
     dfree(h->li_P);
     dfree(h->li_BT);
     dfree(h->li_R);
@@ -495,6 +498,52 @@ bool launch_final(Handle* h, int B, int k, int rsel, int64_t* ids_out, float* d_
 
 constexpr int kRobustCap = 65536;  // rows a robust query ranks within the fp32 error window of rank k
 
+// Workspace of the all-rows selections (robust.hip) for up to rb_F queries
+// at a time: the key of every row is kept per query, at most 512 MB of it.
+bool ensure_robust(Handle* h) {
+    if (h->rb_F) return true;
+    const int64_t n = h->n;
+    const int64_t fit = std::max<int64_t>(1, ((int64_t)512 << 20) / (n * 4));
+    const int F = (int)std::min<int64_t>(64, fit);
+    HIP_OR(hipMalloc(&h->rb_keys, (size_t)F * n * 4), false);
+    HIP_OR(hipMalloc(&h->rb_hist, (size_t)F * 2048 * 4), false);
+    HIP_OR(hipMalloc(&h->rb_sel, (size_t)F * 4 * 8), false);
+    HIP_OR(hipMalloc(&h->rb_cnt, (size_t)F * 4), false);
+    HIP_OR(hipMalloc(&h->rb_cid, (size_t)F * kRobustCap * 4), false);
+    HIP_OR(hipMalloc(&h->rb_rows, (size_t)F * 4), false);
+    HIP_OR(hipMalloc(&h->rb_flags, (size_t)F * 4), false);
+    HIP_OR(hipMalloc(&h->rb_nres, (size_t)F * 4), false);
+    h->rb_F = F;
+    return true;
+}
+
+RobustArgs robust_args(Handle* h, int mode, int F, int k, int64_t* ids_out, float* d_out) {
+    RobustArgs ra{};
+    ra.mode = mode;
+    ra.X32 = h->X32;
+    ra.x32_stride = h->kdim32;
+    ra.n = h->n;
+    ra.d = h->d;
+    ra.Q32 = h->dQ32;
+    ra.rows = h->rb_rows;
+    ra.F = F;
+    ra.k = k;
+    ra.r = k;
+    ra.keys = h->rb_keys;
+    ra.hist = h->rb_hist;
+    ra.sel = h->rb_sel;
+    ra.cnt = h->rb_cnt;
+    ra.cid = h->rb_cid;
+    ra.cap = kRobustCap;
+    const double u = 5.9604644775390625e-8;
+    ra.eps_rel = 4.0 * ((h->d + 15) / 16 + 28) * u;  // finalize's near-tie window
+    ra.out_ids = ids_out;
+    ra.out_d = d_out;
+    ra.flags = h->rb_flags;
+    ra.n_rescored = h->rb_nres;
+    return ra;
+}
+
 // Re-runs queries `qs` of the step just completed (queries in h->dQ32,
 // outputs rows of ids_out / d_out) with the exact all-rows path of
 // robust.hip, a bounded number at a time.  rflags / rnres receive, per
@@ -503,47 +552,13 @@ constexpr int kRobustCap = 65536;  // rows a robust query ranks within the fp32 
 bool run_robust(Handle* h, const std::vector<int32_t>& qs, int k, int64_t* ids_out, float* d_out,
                 std::vector<uint32_t>& rflags, std::vector<uint32_t>& rnres) {
     hipStream_t st = h->st;
-    const int64_t n = h->n;
-    if (!h->rb_F) {
-        // d32 of every row is kept per query: at most 512 MB of it at a time
-        const int64_t fit = std::max<int64_t>(1, ((int64_t)512 << 20) / (n * 4));
-        const int F = (int)std::min<int64_t>(64, fit);
-        HIP_OR(hipMalloc(&h->rb_keys, (size_t)F * n * 4), false);
-        HIP_OR(hipMalloc(&h->rb_hist, (size_t)F * 2048 * 4), false);
-        HIP_OR(hipMalloc(&h->rb_sel, (size_t)F * 4 * 4), false);
-        HIP_OR(hipMalloc(&h->rb_cnt, (size_t)F * 4), false);
-        HIP_OR(hipMalloc(&h->rb_cid, (size_t)F * kRobustCap * 4), false);
-        HIP_OR(hipMalloc(&h->rb_rows, (size_t)F * 4), false);
-        HIP_OR(hipMalloc(&h->rb_flags, (size_t)F * 4), false);
-        HIP_OR(hipMalloc(&h->rb_nres, (size_t)F * 4), false);
-        h->rb_F = F;
-    }
+    if (!ensure_robust(h)) return false;
     rflags.assign(qs.size(), 0u);
     rnres.assign(qs.size(), 0u);
-    const double u = 5.9604644775390625e-8;
     for (size_t f0 = 0; f0 < qs.size(); f0 += (size_t)h->rb_F) {
         const int F = (int)std::min<size_t>((size_t)h->rb_F, qs.size() - f0);
         HIP_OR(hipMemcpyAsync(h->rb_rows, qs.data() + f0, (size_t)F * 4, hipMemcpyHostToDevice, st), false);
-        RobustArgs ra{};
-        ra.X32 = h->X32;
-        ra.x32_stride = h->kdim32;
-        ra.n = n;
-        ra.d = h->d;
-        ra.Q32 = h->dQ32;
-        ra.rows = h->rb_rows;
-        ra.F = F;
-        ra.k = k;
-        ra.keys = h->rb_keys;
-        ra.hist = h->rb_hist;
-        ra.sel = h->rb_sel;
-        ra.cnt = h->rb_cnt;
-        ra.cid = h->rb_cid;
-        ra.cap = kRobustCap;
-        ra.eps_rel = 4.0 * ((h->d + 15) / 16 + 28) * u;  // finalize's near-tie window
-        ra.out_ids = ids_out;
-        ra.out_d = d_out;
-        ra.flags = h->rb_flags;
-        ra.n_rescored = h->rb_nres;
+        const RobustArgs ra = robust_args(h, ROBUST_EXACT, F, k, ids_out, d_out);
         HIP_OR(launch_robust(ra, st), false);
         HIP_OR(hipMemcpyAsync(rflags.data() + f0, h->rb_flags, (size_t)F * 4, hipMemcpyDeviceToHost, st), false);
         HIP_OR(hipMemcpyAsync(rnres.data() + f0, h->rb_nres, (size_t)F * 4, hipMemcpyDeviceToHost, st), false);
@@ -2276,17 +2291,67 @@ void* gf_create(const float* train, long n, long dim) {
 }
 
 int gf_query(void* handle, const float* q, int k, int r, long long* out_ids) {
+    g_err[0] = 0;
     Handle* h = static_cast<Handle*>(handle);
     if (!h) {
         set_err("null handle");
         return -1;
     }
+    if (!q || !out_ids || k <= 0) {
+        set_err("bad arguments: k=%d", k);
+        return -1;
+    }
+    if (h->kind != SCAN_SQ8 || h->ivf) {
+        set_err("gf_query needs a gf_create (per-dim SQ8) handle");
+        return -1;
+    }
+    if (k > 2048) {
+        set_err("k=%d exceeds 2048", k);
+        return -1;
+    }
     // gpu_filter.cu:145-146: r clamped to [k, n]
     if (r < k) r = k;
     if (r > h->n) r = (int)h->n;
-    std::vector<float> dd(k);
+    const int out_k = std::min(k, r);
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipStream_t st = h->st;
+    if (!ensure_ws(h, 1, k) || !ensure_robust(h)) return -1;
+    RobustArgs ra = robust_args(h, ROBUST_GPU_FILTER, 1, k, h->d_ids, h->d_dist);
+    ra.r = r;
+    ra.codes = static_cast<const uint8_t*>(h->Xs);
+    ra.kdim = h->kdim;
+    ra.tiled = h->tiled;
+    ra.scale = h->sq_scale;
+    ra.offset = h->sq_offset;
+    // the r rows of the cut are collected whole: the shared list holds
+    // rb_F * kRobustCap ids, a larger r takes a list of its own
+    ra.cap = (int64_t)h->rb_F * kRobustCap;
+    if (r > ra.cap) {
+        if (h->gf_cap < r) {
+            dfree(h->gf_cid);
+            h->gf_cid = nullptr;
+            h->gf_cap = 0;
+            HIP_OR(hipMalloc(&h->gf_cid, (size_t)r * 4), -1);
+            h->gf_cap = r;
+        }
+        ra.cid = h->gf_cid;
+        ra.cap = h->gf_cap;
+    }
+    const int32_t row0 = 0;
+    uint32_t flag = 0;
+    HIP_OR(hipMemcpyAsync(h->dQ32, q, (size_t)h->d * 4, hipMemcpyHostToDevice, st), -1);
+    HIP_OR(hipMemcpyAsync(h->rb_rows, &row0, 4, hipMemcpyHostToDevice, st), -1);
+    HIP_OR(launch_robust(ra, st), -1);
     static_assert(sizeof(long long) == sizeof(int64_t), "");
-    return batch_impl(h, q, 1, k, r, reinterpret_cast<int64_t*>(out_ids), dd.data(), false, 0);
+    HIP_OR(hipMemcpyAsync(out_ids, h->d_ids, (size_t)out_k * 8, hipMemcpyDeviceToHost, st), -1);
+    HIP_OR(hipMemcpyAsync(&flag, h->rb_flags, 4, hipMemcpyDeviceToHost, st), -1);
+    HIP_OR(hipStreamSynchronize(st), -1);
+    if (flag) {
+        set_err("gf_query: collected more than %lld rows", (long long)ra.cap);
+        return -1;
+    }
+    h->ndist += r;  // gpu_filter.cu:171
+    return out_k;
 }
 
 long long gf_ndist(void* handle) { return handle ? (long long)static_cast<Handle*>(handle)->ndist.load() : 0; }

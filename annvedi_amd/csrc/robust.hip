@@ -1,14 +1,12 @@
-// Exact top-k over every row for queries a regular step flagged (robust.h).
+// Selections over every row for a few queries (robust.h).
 //
-// Rare path: the regular pipeline bounds its candidate lists (65536 keys per
-// query) and its re-score set (2048 rows); adversarial row orders, thousands
-// of duplicate vectors or long fp32 near-tie runs exceed those bounds.  This
-// path ranks a query against all n rows with no such bound: the fp32 value
-// of every row (reference summation order, the value finalize reports), a
-// device-wide radix select of the rank-k value, a collection of the rows
-// within the fp32 error window of it, and a chunked fp64 ranking by
-// (distance, id) -- the order src/linear_index.cpp:110-118 defines with the
-// value taken as the exact distance.
+// Rare paths with no bounded candidate list: the exact answer of queries the
+// regular pipeline flagged (adversarial row orders, thousands of duplicate
+// vectors, long fp32 near-tie runs), and the reference GPU contestant's
+// ranking chain for its drop-in entry point (gf_query).  Both rank a query
+// against all n rows: a key per row, a device-wide radix select of the
+// rank-k (or rank-r) value over the 64-bit (key, row) image, a collection of
+// the rows at or below it, and a chunked ranking in LDS by (value, row).
 #include "robust.h"
 #include "select_dev.h"
 
@@ -17,19 +15,29 @@ namespace {
 constexpr int kChunk = 2048;      // collected rows ranked per LDS merge
 constexpr int kMerge = 2 * kChunk;
 
+// 64-bit ordered image of (key, row): ascending order of (key, row).
+__device__ inline uint64_t comp_key(float key, int64_t row) {
+    return ((uint64_t)f2key(key) << 32) | (uint64_t)(uint32_t)row;
+}
+
+__device__ inline int64_t rank_of(const RobustArgs& a) {
+    const int64_t want = a.mode == ROBUST_GPU_FILTER ? (int64_t)a.r : (int64_t)a.k;
+    return want < a.n ? want : a.n;
+}
+
 __global__ __launch_bounds__(kThreads) void rb_init(RobustArgs a) {
     const int f = blockIdx.x;
     for (int b = threadIdx.x; b < kBins; b += kThreads) a.hist[(int64_t)f * kBins + b] = 0u;
     if (threadIdx.x == 0) {
-        a.sel[f * 4 + 0] = 0u;  // prefix
-        a.sel[f * 4 + 1] = 0u;  // mask
-        a.sel[f * 4 + 2] = (uint32_t)((int64_t)a.k < a.n ? (int64_t)a.k : a.n);  // rank (1-based)
-        a.sel[f * 4 + 3] = 0u;
+        a.sel[f * 4 + 0] = 0ull;  // prefix
+        a.sel[f * 4 + 1] = 0ull;  // mask
+        a.sel[f * 4 + 2] = (uint64_t)rank_of(a);  // rank (1-based)
+        a.sel[f * 4 + 3] = 0ull;
         a.cnt[f] = 0u;
     }
 }
 
-// d32 of every row: one 16-lane group per row, two rows at a time.
+// Exact mode: d32 of every row, one 16-lane group per row, two at a time.
 __global__ __launch_bounds__(kThreads) void rb_keys(RobustArgs a) {
     extern __shared__ float qs[];
     const int f = blockIdx.y;
@@ -51,19 +59,55 @@ __global__ __launch_bounds__(kThreads) void rb_keys(RobustArgs a) {
     }
 }
 
-// One radix pass: histogram of the next digit of the keys that match the
-// prefix so far, LDS first, then added into the query's global histogram.
+// gpu_filter mode: the reference's SQ8 key of every row (gpu_filter.cu:52-63),
+// one row per thread, dims in order; the 64 code bytes of a chunk are read
+// as four 16-byte loads (contiguous across the 16 rows of a tile).
+__global__ __launch_bounds__(kThreads) void rb_gf_keys(RobustArgs a) {
+    extern __shared__ float sm[];
+    float* qsh = sm;          // q - offset (gpu_filter.cu:148-149)
+    float* sc = sm + a.d;     // scale
+    const int f = blockIdx.y;
+    const float* q = a.Q32 + (int64_t)a.rows[f] * a.d;
+    for (int j = threadIdx.x; j < a.d; j += kThreads) {
+        qsh[j] = __fsub_rn(q[j], a.offset[j]);
+        sc[j] = a.scale[j];
+    }
+    __syncthreads();
+    float* keys = a.keys + (int64_t)f * a.n;
+    const int64_t stride = a.kdim;  // bytes per row
+    for (int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x; row < a.n; row += (int64_t)gridDim.x * kThreads) {
+        float acc = 0.f;
+        for (int c0 = 0; c0 < a.d; c0 += 64) {
+            const int64_t off = a.tiled ? tiled_off(row, c0, stride, 64) : row * stride + c0;
+            const uint4* p = reinterpret_cast<const uint4*>(a.codes + off);
+            uint4 w[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) w[u] = p[u];
+            const uint8_t* b = reinterpret_cast<const uint8_t*>(w);
+            const int m = a.d - c0 < 64 ? a.d - c0 : 64;
+            for (int j = 0; j < m; ++j) {
+                const float v = fmaf((float)b[j], sc[c0 + j], -qsh[c0 + j]);
+                acc = fmaf(v, v, acc);
+            }
+        }
+        keys[row] = acc;
+    }
+}
+
+// One radix pass over the (key, row) image: histogram of the digit at
+// [sft, sft + width) of the values that match the prefix so far, LDS first,
+// then added into the query's global histogram.
 __global__ __launch_bounds__(kThreads) void rb_hist(RobustArgs a, int sft, int width) {
     __shared__ uint32_t h[kBins];
     const int f = blockIdx.y;
     for (int b = threadIdx.x; b < kBins; b += kThreads) h[b] = 0u;
     __syncthreads();
-    const uint32_t prefix = a.sel[f * 4 + 0], mask = a.sel[f * 4 + 1];
-    const uint32_t wmask = (1u << width) - 1u;
+    const uint64_t prefix = a.sel[f * 4 + 0], mask = a.sel[f * 4 + 1];
+    const uint64_t wmask = (1ull << width) - 1ull;
     const float* keys = a.keys + (int64_t)f * a.n;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * kThreads) {
-        const uint32_t k = f2key(keys[i]);
-        if ((k & mask) == prefix) atomicAdd(&h[(k >> sft) & wmask], 1u);
+        const uint64_t c = comp_key(keys[i], i);
+        if ((c & mask) == prefix) atomicAdd(&h[(uint32_t)((c >> sft) & wmask)], 1u);
     }
     __syncthreads();
     uint32_t* g = a.hist + (int64_t)f * kBins;
@@ -72,8 +116,9 @@ __global__ __launch_bounds__(kThreads) void rb_hist(RobustArgs a, int sft, int w
 }
 
 // The digit holding the rank; clears the histogram for the next pass.  After
-// the last pass the prefix is the rank-k key and the collection threshold
-// t * (1 + eps) (rounded up) is stored.
+// the last pass: exact mode stores the collection threshold t * (1 + eps)
+// (rounded up) of the rank-k key t; gpu_filter mode the rank-r (key, row)
+// value itself.
 __global__ __launch_bounds__(kThreads) void rb_pick(RobustArgs a, int sft, int width, int last) {
     __shared__ uint32_t h[kBins];
     __shared__ uint32_t sh[8];
@@ -86,35 +131,53 @@ __global__ __launch_bounds__(kThreads) void rb_pick(RobustArgs a, int sft, int w
     __syncthreads();
     int bin;
     uint32_t below;
-    find_bin(h, a.sel[f * 4 + 2], sh, &bin, &below);
+    find_bin(h, (uint32_t)a.sel[f * 4 + 2], sh, &bin, &below);
     if (threadIdx.x == 0) {
-        const uint32_t wmask = (1u << width) - 1u;
-        const uint32_t prefix = a.sel[f * 4 + 0] | ((uint32_t)bin << sft);
+        const uint64_t wmask = (1ull << width) - 1ull;
+        const uint64_t prefix = a.sel[f * 4 + 0] | ((uint64_t)bin << sft);
         a.sel[f * 4 + 0] = prefix;
         a.sel[f * 4 + 1] |= wmask << sft;
         a.sel[f * 4 + 2] -= below;
         if (last) {
-            const float t = key2f(prefix);
-            float tc = t;
-            if (t == t && t < __builtin_inff()) {
-                tc = (float)((double)t * (1.0 + a.eps_rel));
-                tc = next_up(tc < t ? t : tc);
+            if (a.mode == ROBUST_GPU_FILTER) {
+                a.sel[f * 4 + 3] = prefix;
+            } else {
+                const float t = key2f((uint32_t)(prefix >> 32));
+                float tc = t;
+                if (t == t && t < __builtin_inff()) {
+                    tc = (float)((double)t * (1.0 + a.eps_rel));
+                    tc = next_up(tc < t ? t : tc);
+                }
+                a.sel[f * 4 + 3] = (uint64_t)__builtin_bit_cast(uint32_t, tc);
             }
-            a.sel[f * 4 + 3] = __builtin_bit_cast(uint32_t, tc);
         }
     }
 }
 
 __global__ __launch_bounds__(kThreads) void rb_collect(RobustArgs a) {
     const int f = blockIdx.y;
-    const float tc = __builtin_bit_cast(float, a.sel[f * 4 + 3]);
+    const uint64_t lim = a.sel[f * 4 + 3];
+    const float tc = __builtin_bit_cast(float, (uint32_t)lim);
+    const bool gf = a.mode == ROBUST_GPU_FILTER;
     const float* keys = a.keys + (int64_t)f * a.n;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * kThreads) {
-        if (keys[i] <= tc) {
+        const bool take = gf ? comp_key(keys[i], i) <= lim : keys[i] <= tc;
+        if (take) {
             const uint32_t pos = atomicAdd(&a.cnt[f], 1u);
-            if (pos < (uint32_t)a.cap) a.cid[(int64_t)f * a.cap + pos] = (uint32_t)i;
+            if ((int64_t)pos < a.cap) a.cid[(int64_t)f * a.cap + pos] = (uint32_t)i;
         }
     }
+}
+
+// gpu_filter.cu:161-170: acc += v * v in float, dims in order, the product
+// rounded on its own (no fused multiply-add).
+__device__ inline float l2_scalar_f32(const float* x, const float* q, int d) {
+    float acc = 0.f;
+    for (int j = 0; j < d; ++j) {
+        const float v = __fsub_rn(x[j], q[j]);
+        acc = __fadd_rn(acc, __fmul_rn(v, v));
+    }
+    return acc;
 }
 
 // Ascending bitonic sort of n (power of two) entries by (k64, id), moving v.
@@ -143,21 +206,24 @@ __device__ void bitonic_pairs(uint64_t* k64, uint32_t* id, float* v, int n) {
     }
 }
 
-// Top-k of the collected rows by (fp64 distance, id): chunks of up to 2048
-// rows are merged into the running top-k in LDS.
+// Top-k of the collected rows, chunks of up to 2048 rows merged into the
+// running top-k in LDS: exact mode by (fp64 distance, row) reporting d32;
+// gpu_filter mode by (scalar fp32 value, row) reporting that value.
 __global__ __launch_bounds__(kThreads) void rb_final(RobustArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     uint64_t* k64 = reinterpret_cast<uint64_t*>(smem);
     uint32_t* id = reinterpret_cast<uint32_t*>(smem + kMerge * sizeof(uint64_t));
     float* v = reinterpret_cast<float*>(smem + kMerge * (sizeof(uint64_t) + sizeof(uint32_t)));
     float* qs = v + kMerge;
+    const bool gf = a.mode == ROBUST_GPU_FILTER;
     const int f = blockIdx.x;
     const int64_t row = a.rows[f];
     for (int j = threadIdx.x; j < a.d; j += kThreads) qs[j] = a.Q32[row * a.d + j];
-    const int kk = (int)((int64_t)a.k < a.n ? (int64_t)a.k : a.n);
+    const int64_t lim = gf ? ((int64_t)a.r < a.n ? (int64_t)a.r : a.n) : a.n;
+    const int kk = (int)((int64_t)a.k < lim ? (int64_t)a.k : lim);
     uint32_t total = a.cnt[f];
     uint32_t flag = 0u;
-    if (total > (uint32_t)a.cap) {
+    if ((int64_t)total > a.cap) {
         flag = 1u;
         total = (uint32_t)a.cap;
     }
@@ -169,10 +235,17 @@ __global__ __launch_bounds__(kThreads) void rb_final(RobustArgs a) {
         const int m = (int)(total - c0 < (uint32_t)kChunk ? total - c0 : (uint32_t)kChunk);
         for (int j = threadIdx.x; j < m; j += kThreads) {
             const uint32_t rid = cid[c0 + j];
-            const double d64 = l2_f64(a.X32 + (int64_t)rid * a.x32_stride, qs, a.d);
-            k64[r + j] = __builtin_bit_cast(uint64_t, d64);  // non-negative: bits order as values
+            const float* x = a.X32 + (int64_t)rid * a.x32_stride;
+            if (gf) {
+                const float dv = l2_scalar_f32(x, qs, a.d);
+                k64[r + j] = (uint64_t)f2key(dv);
+                v[r + j] = dv;
+            } else {
+                const double d64 = l2_f64(x, qs, a.d);
+                k64[r + j] = __builtin_bit_cast(uint64_t, d64);  // non-negative: bits order as values
+                v[r + j] = keys[rid];
+            }
             id[r + j] = rid;
-            v[r + j] = keys[rid];
         }
         int np = 1;
         while (np < r + m) np <<= 1;
@@ -210,6 +283,8 @@ size_t robust_final_smem_bytes(int d) {
 
 hipError_t launch_robust(const RobustArgs& a, hipStream_t st) {
     if (a.F <= 0 || a.n <= 0) return hipSuccess;
+    if (a.mode == ROBUST_GPU_FILTER && (!a.codes || (a.kdim & 63) || a.d > a.kdim || a.r < 1))
+        return hipErrorInvalidValue;
     const int64_t gb = (a.n + 4095) / 4096;
     const unsigned gx = (unsigned)(gb < 1024 ? gb : 1024);
     static const bool attr = [] {  // the merge buffers take more than the default 64 KB of LDS
@@ -219,11 +294,21 @@ hipError_t launch_robust(const RobustArgs& a, hipStream_t st) {
     }();
     (void)attr;
     ANNMI_LAUNCH(rb_init, dim3((unsigned)a.F), dim3(kThreads), 0, st, a);
-    ANNMI_LAUNCH(rb_keys, dim3(gx, (unsigned)a.F), dim3(kThreads), (size_t)a.d * sizeof(float), st, a);
-    const int sft[3] = {21, 10, 0}, wid[3] = {11, 11, 10};
-    for (int p = 0; p < 3; ++p) {
+    if (a.mode == ROBUST_GPU_FILTER) {
+        const int64_t gr = (a.n + kThreads - 1) / kThreads;
+        const unsigned gk = (unsigned)(gr < 4096 ? gr : 4096);
+        ANNMI_LAUNCH(rb_gf_keys, dim3(gk, (unsigned)a.F), dim3(kThreads), (size_t)a.d * 2 * sizeof(float), st, a);
+    } else {
+        ANNMI_LAUNCH(rb_keys, dim3(gx, (unsigned)a.F), dim3(kThreads), (size_t)a.d * sizeof(float), st, a);
+    }
+    // digits of the 64-bit (key, row) image, most significant first: the key
+    // alone (three digits) for the exact mode's window, key and row (six)
+    // for gpu_filter's exact rank-r cut
+    const int sft[6] = {53, 42, 32, 21, 10, 0}, wid[6] = {11, 11, 10, 11, 11, 10};
+    const int passes = a.mode == ROBUST_GPU_FILTER ? 6 : 3;
+    for (int p = 0; p < passes; ++p) {
         ANNMI_LAUNCH(rb_hist, dim3(gx, (unsigned)a.F), dim3(kThreads), 0, st, a, sft[p], wid[p]);
-        ANNMI_LAUNCH(rb_pick, dim3((unsigned)a.F), dim3(kThreads), 0, st, a, sft[p], wid[p], p == 2 ? 1 : 0);
+        ANNMI_LAUNCH(rb_pick, dim3((unsigned)a.F), dim3(kThreads), 0, st, a, sft[p], wid[p], p == passes - 1 ? 1 : 0);
     }
     ANNMI_LAUNCH(rb_collect, dim3(gx, (unsigned)a.F), dim3(kThreads), 0, st, a);
     ANNMI_LAUNCH(rb_final, dim3((unsigned)a.F), dim3(kThreads), robust_final_smem_bytes(a.d), st, a);

@@ -26,6 +26,7 @@
 //   orc_sq4_train/encode     submission/src/hnsw.cpp:147-183 (nibble layout)
 //   orc_sq4_l2               submission/src/simd.h:292-318 (AVX-512 branch, no threshold)
 //   orc_f32_direct_l2        gpu_filter.cu:161-170 rerank loop (scalar acc += v*v)
+//   orc_gf_keys              gpu_filter.cu:52-63 (serial fmaf SQ8 key per row)
 //   orc_kmeans_init          lsh_index_optimized.cpp:553-561 (initial centroid rows)
 //   orc_ivf_kmeans           lsh_index_optimized.cpp:551-616, :32-73 (k-means, one thread)
 //
@@ -240,6 +241,21 @@ float orc_f32_direct_l2(const float* a, const float* b, int64_t dim) {
         acc = acc + sq;
     }
     return acc;
+}
+
+// gpu_filter.cu:52-63 (sq8pd_scan): the key of each of n dim-ordered rows,
+// v = fma(code, scale, -q_shift) and acc = fma(v, v, acc) over dims in order
+// (one CUDA thread per row, serial fmaf).  codes row-major [n x dim].
+void orc_gf_keys(const uint8_t* codes, int64_t n, int64_t dim, const float* q_shift, const float* scale,
+                 float* keys) {
+    for (int64_t i = 0; i < n; ++i) {
+        float acc = 0.0f;
+        for (int64_t d = 0; d < dim; ++d) {
+            float v = fmaf((float)codes[i * dim + d], scale[d], -q_shift[d]);
+            acc = fmaf(v, v, acc);
+        }
+        keys[i] = acc;
+    }
 }
 
 // ---------------------------------------------------------------- SQ8 ----

@@ -41,6 +41,12 @@ Parity pins (each a check a plausible mistake would fail):
 * ``f32_direct_l2`` (gpu_filter.cu:161-170): integer-valued rows (every sum
   exact), a sum that only the sequential order absorbs, a product that FMA
   contraction would not round.
+* ``gf_keys``, ``gf_query_reference`` (gpu_filter.cu:52-63, 142-176): the
+  whole chain brute-forced in integer arithmetic on rows whose codes equal
+  their values (every key and re-rank exact), duplicates for the stable
+  order of the cut and the (value, id) order of the answer; a fused
+  multiply-add a separate product would round differently; a sequential
+  chain a tree would not absorb.
 """
 from __future__ import annotations
 
@@ -106,6 +112,8 @@ def lib():
         L.orc_sq4_l2.argtypes = [_u8p, _f32p, _f32p, _i64]
         L.orc_kmeans_init.restype = None
         L.orc_kmeans_init.argtypes = [_i64, _i32, ctypes.c_uint32, _i64p, _i64p]
+        L.orc_gf_keys.restype = None
+        L.orc_gf_keys.argtypes = [_u8p, _i64, _i64, _f32p, _f32p, _f32p]
         L.orc_ivf_kmeans.restype = None
         L.orc_ivf_kmeans.argtypes = [_f32p, _i64, _i64, _i32, _i32, ctypes.c_uint32, _f32p,
                                      np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
@@ -210,6 +218,36 @@ def sq8_l2(code_row, q_shifted, scale) -> np.float32:
 def sq8_l2_f64(code_row, q_shifted, scale) -> float:
     c = np.ascontiguousarray(code_row, dtype=np.uint8)
     return float(lib().orc_sq8_l2_f64(c, _f32(q_shifted), _f32(scale), c.shape[0]))
+
+
+def gf_keys(codes, q_shift, scale):
+    """The reference GPU contestant's SQ8 key of every row (gpu_filter.cu:52-63)."""
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    n, d = codes.shape
+    out = np.empty(n, dtype=np.float32)
+    lib().orc_gf_keys(codes, n, d, _f32(q_shift), _f32(scale), out)
+    return out
+
+
+def gf_query_reference(base, q, k, r):
+    """competitors/gpu_filter/src/gpu_filter.cu:79-176 for one query: the
+    per-dim SQ8 encoder of gf_create (:99-127, lrintf, constant dim offset
+    min); r clamped to [k, n] (:145-146); keys of every row (:52-63); the
+    first r rows of the stable ascending sort of (key, row) (CUB SortPairs
+    over idx = 0..n-1, :154-155); each re-ranked by the scalar float sum
+    acc += v * v, v = row - q (:161-170); partial_sort by (value, row) and
+    the first min(k, r) rows (:172-175).  Returns (ids int64, values float32)."""
+    base, q = _f32(base), _f32(q)
+    n = base.shape[0]
+    r = max(int(r), int(k))
+    r = min(r, n)
+    scale, offset = sq8_train(base, 1)
+    codes = sq8_encode(base, scale, offset, 1)
+    keys = gf_keys(codes, (q - offset).astype(np.float32), scale)
+    cand = np.argsort(keys, kind="stable")[:r]
+    vals = np.array([f32_direct_l2(base[c], q) for c in cand], dtype=np.float32)
+    order = np.lexsort((cand, vals))[:min(int(k), r)]
+    return cand[order].astype(np.int64), vals[order]
 
 
 # ------------------------------------------------------------------ SQ4 ----

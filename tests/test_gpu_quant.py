@@ -47,22 +47,61 @@ def test_rescore_everything_equals_exact(mode):
     np.testing.assert_array_equal(ids, ref_ids)
 
 
-def test_gpu_filter_abi_drop_in():
-    """Bind gf_* exactly as competitors/gpu_filter/algorithm.py:6-15 does."""
+def _gf(L, h, q, k, r):
+    out = np.full(k, -7, dtype=np.int64)
+    got = L.gf_query(h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), k, r,
+                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+    return got, out
+
+
+def test_gpu_filter_abi_returns_the_reference_chain_ids():
+    """Bind gf_* exactly as competitors/gpu_filter/algorithm.py:6-15 does; the
+    ids equal the reference chain's (gpu_filter.cu:142-176, restated in the
+    oracle): top r by the fp32 SQ8 key in stable order, scalar fp32 re-rank,
+    (value, id) order -- at the scenario values r = 150 and 500, past the
+    2048-row re-score of the other modes, and at r = n."""
     L = _lib.load()
-    base, Q = data.workload("exact-1m-384", n=20000, n_queries=8)
+    base, Q = data.workload("exact-1m-384", n=20000, n_queries=6)
     h = L.gf_create(base.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), base.shape[0], base.shape[1])
     assert h, L.gf_last_error()
-    ref_ids, _ = O.exact_topk_f64(base, Q, 100)
-    out = np.empty(100, dtype=np.int64)
-    recs = []
-    for i, q in enumerate(Q):
-        got = L.gf_query(h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 100, 500,
-                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
-        assert got == 100
-        recs.append(len(set(out.tolist()) & set(ref_ids[i].tolist())) / 100)
-    assert np.mean(recs) >= 0.99
-    assert L.gf_ndist(h) >= 8 * 500
+    total_r = 0
+    for r in (150, 500, 5000, 20000):
+        print(f"[case] gf_query r={r}", flush=True)
+        for q in Q:
+            got, out = _gf(L, h, q, 100, r)
+            assert got == 100, L.gf_last_error()
+            want, _ = O.gf_query_reference(base, q, 100, r)
+            np.testing.assert_array_equal(out, want)
+            total_r += r
+    # r below k is raised to k (gpu_filter.cu:145), and counted so
+    got, out = _gf(L, h, Q[0], 100, 30)
+    assert got == 100
+    np.testing.assert_array_equal(out, O.gf_query_reference(base, Q[0], 100, 30)[0])
+    total_r += 100
+    assert L.gf_ndist(h) == total_r  # n_distances += r per query (:171)
+    L.gf_free(h)
+
+
+def test_gpu_filter_abi_exact_integer_case():
+    """Integer rows whose keys and re-ranks are exact, duplicates for the
+    stable cut and the (value, id) order, k > n clamps."""
+    L = _lib.load()
+    rng = np.random.Generator(np.random.PCG64(9))
+    n, d = 3000, 64
+    x = rng.integers(0, 511, size=(n, d)).astype(np.float32)
+    x[0] = 0
+    x[1] = 510
+    x[100:400] = x[55]
+    h = L.gf_create(x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n, d)
+    assert h, L.gf_last_error()
+    for j in range(4):
+        q = (x[55] if j == 0 else rng.integers(0, 511, size=d)).astype(np.float32)
+        for k, r in [(100, 150), (100, 500), (250, 250), (5000, 4000)]:
+            got, out = _gf(L, h, q, min(k, 2048), r)
+            kk = min(min(k, 2048), max(r, min(k, 2048)), n)
+            assert got == kk, L.gf_last_error()
+            want, _ = O.gf_query_reference(x, q, min(k, 2048), r)
+            np.testing.assert_array_equal(out[:got], want)
     L.gf_free(h)
 
 

@@ -250,3 +250,56 @@ def test_f32_direct_l2_closed_forms():
     got = O.f32_direct_l2(np.array([1.0, v], np.float32), np.zeros(2, np.float32))
     assert got == np.float32(2.0 + 3.0 * 2.0 ** -12)
     assert got != np.float32(2.0 + 3.0 * 2.0 ** -12 + 2.0 ** -22)
+
+
+# ------------------------------------------------------------ gpu_filter ----
+def _gf_integer_case(seed):
+    """Rows of even and odd integers in [0, 510] with both ends present in
+    every dim: scale is exactly 2 and offset 0, so a code is x / 2 rounded
+    half to even (lrintf), and every key, re-rank and sum below is an exact
+    integer; a block of duplicated rows makes ties."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n, d = 80, 8
+    x = rng.integers(0, 511, size=(n, d)).astype(np.float32)
+    x[0] = 0
+    x[1] = 510
+    x[20:30] = x[7]
+    q = rng.integers(0, 511, size=d).astype(np.float32)
+    return x, q
+
+
+def _gf_brute_force(x, q, k, r):
+    xi, qi = x.astype(np.int64), q.astype(np.int64)
+    code = np.rint(x / 2).astype(np.int64)           # half to even
+    key = ((2 * code - qi) ** 2).sum(1)               # sq8pd_scan, exact here
+    r = min(max(r, k), x.shape[0])
+    cand = np.argsort(key, kind="stable")[:r]         # stable sort: ties by row
+    val = ((xi[cand] - qi) ** 2).sum(1)               # the re-rank, exact here
+    order = np.lexsort((cand, val))[:min(k, r)]       # partial_sort of (value, id)
+    return cand[order], val[order]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_gf_chain_brute_force_on_exact_integers(seed):
+    x, q = _gf_integer_case(seed)
+    s, o = O.sq8_train(x, 1)
+    assert np.all(s == 2.0) and np.all(o == 0.0)
+    for k, r in [(10, 10), (10, 25), (30, 40), (10, 80), (50, 20)]:
+        ids, vals = O.gf_query_reference(x, q, k, r)
+        want_ids, want_vals = _gf_brute_force(x, q, k, r)
+        np.testing.assert_array_equal(ids, want_ids, err_msg=f"k={k} r={r}")
+        np.testing.assert_array_equal(vals, want_vals.astype(np.float32))
+
+
+def test_gf_key_is_a_serial_fused_chain():
+    # fused: 3 * (1 + 2^-23) - 3 = 3 * 2^-23 exactly; a rounded product first
+    # gives 3 + 2^-21 - 3 = 2^-21
+    key = O.gf_keys(np.array([[3]], np.uint8), np.array([3.0], np.float32),
+                    np.array([1.0 + 2.0 ** -23], np.float32))[0]
+    assert key == np.float32((3 * 2.0 ** -23) ** 2)
+    # dims in order: 4096^2 = 2^24 first absorbs each later +1 (tie to even);
+    # a tree of partial sums would not
+    codes = np.array([[0] + [1] * 15], np.uint8)
+    qs = np.array([-4096.0] + [0.0] * 15, np.float32)
+    sc = np.ones(16, np.float32)
+    assert O.gf_keys(codes, qs, sc)[0] == np.float32(2.0 ** 24)
