@@ -17,6 +17,13 @@ shapes on the fitted index (up to ``max_batch`` queries) before the first
 query; ``index_file`` names a saved index (``Index.save``) to load instead of
 building one, and is written after the build when it does not exist yet.
 
+``backend: gpu`` is the reference hybrid's GPU backend
+(competitors/hybrid/algorithm.py:54-95): the per-dim SQ8 index of
+``gf_create`` and the reference GPU contestant's ranking chain per query
+(``gf_query``: top ``r`` by the SQ8 key, default 150, fp32 re-rank), its
+answers identical to the reference's; the HNSW parameters of that backend
+block (M, ef_construction, mode, heuristic, reorder, ef) are ignored.
+
 ``get_n_distances`` counts full-precision distance evaluations: N per query
 in exact mode (src/linear_index.cpp:108), the re-scored rows per query in the
 quantized modes (gpu_filter.cu:171, "only full Euclidean distances count").
@@ -39,6 +46,7 @@ class Algorithm:
         self._rerank = 0
         self._nprobe = 0
         self._compat = False
+        self._gf = False
 
     def fit(self, train: np.ndarray, **index_params) -> None:
         ds_size = index_params.pop("ds_size", None)
@@ -47,6 +55,13 @@ class Algorithm:
         self._backend = str(index_params.pop("backend", "mi355x")).lower()
         if self._backend not in BACKENDS:
             raise ValueError(f"backend {self._backend!r} is not served by this plugin ({BACKENDS})")
+        self._gf = self._backend == "gpu"
+        if self._gf:
+            # the hybrid's gpu backend: gf_create's index, gf_query's chain
+            for unused in ("M", "ef_construction", "mode", "heuristic", "reorder", "ef"):
+                index_params.pop(unused, None)
+            index_params["mode"] = "sq8"
+            index_params["sq_variant"] = 1
         # mode 'linear_index': the reference's LinearIndex.query semantics bit
         # for bit (early-abandon values beyond rank 64 included)
         mode = str(index_params.pop("mode", "exact")).lower()
@@ -88,6 +103,8 @@ class Algorithm:
             raise RuntimeError(f"query must be a 1-D vector of length {self._index.d}")
         if self._compat:
             return self._index.search_linear_index(q[None, :], int(k))[0][0]
+        if self._gf:
+            return self._index.search_gpu_filter(q, int(k), int(query_params.get("r", 150)))
         r, npb = self._qp(query_params)
         ids, _ = self._index.search(q[None, :], int(k), rerank=r, nprobe=npb)
         return ids[0]
@@ -103,6 +120,13 @@ class Algorithm:
             if _is_device(queries):
                 queries = queries.cpu().numpy()
             return self._index.search_linear_index(np.asarray(queries, dtype=np.float32), int(k))
+        if self._gf:  # the reference contestant answers one query per call
+            if _is_device(queries):
+                queries = queries.cpu().numpy()
+            Q = np.asarray(queries, dtype=np.float32)
+            ids = np.stack([self._index.search_gpu_filter(q, int(k), int(query_params.get("r", 150))) for q in Q]) \
+                if len(Q) else np.empty((0, min(int(k), self._index.n)), dtype=np.int64)
+            return ids, None
         r, npb = self._qp(query_params)
         if _is_device(queries):
             return self._index.search_device(queries, int(k), rerank=r, nprobe=npb)

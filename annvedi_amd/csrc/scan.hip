@@ -1356,6 +1356,261 @@ hipError_t launch_wide32_t(const ScanArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// SQ8 wide scan on 32 x 32 x 32 int8 MFMA tiles with a lean epilogue (an
+// autotuner shape, kWideCfg32R4W4 / kWideCfg32R2W4 for SQ8).
+//
+// A wave tile is 128 queries x 32 * RS rows: each query fragment read from
+// LDS feeds RS MFMAs and each row fragment four.  The epilogue keeps the
+// register footprint small: the integer-dot prefilter (the accumulators
+// start at -D_r, so max(dot - D_r) against one bound per query is necessary
+// for a pass) marks query tiles that may pass in some lane; only those tiles
+// are written, one 32 x 32 tile at a time, to a 4 KB LDS slot of the wave and
+// tested exactly in a loop that is not unrolled, so the rare appends do not
+// multiply the kernel's code and live ranges (the other 32 x 32 kernel,
+// with its epilogue unrolled over every accumulator, spills 30-200 VGPRs
+// in every SQ8 shape; this one spills none at RS <= 2).  The exact test is
+// the one of the other wide kernels (dist = xn + qv - 2 qs (dot), the same
+// integer dot), so every shape appends the same candidates.  Prefilter mode
+// 3 only (the default for integer codes); the tiled row layout only.
+//
+// Measured (MI355X, 10M x 384, 4096 queries, largest launch of a step):
+// 1913 TOP/s at RS = 2 on 8 waves against 1877 for the default 16 x 16
+// kernel, with the MFMA pipe busy for the same cycles (SQ_VALU_MFMA_BUSY_CYCLES
+// 8.9e9 for both: the 32 x 32 form halves the MFMA instructions, not the
+// MFMA time) and no LDS bank conflicts (1.1e9 conflict cycles in the
+// default kernel); its smaller launches are slower, so a whole step is not
+// (20.2 vs 18.0 ms of scan).  One wave a SIMD (RS = 2 or 4 on 4 waves, the
+// accumulators in AGPRs) measured 1470 / 855 TOP/s: a single wave does not
+// hide the row-load latency with a ring of 4 K steps.
+template <int RS, int D, int NW>
+__global__ __launch_bounds__(64 * NW) void scan_i8x32_kernel(ScanArgs a, int qtiles, int64_t chunk_rows, int64_t nrc,
+                                                              int qcap) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr int QS = 4, QT = 128;
+    typedef W32SQ8 W;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int h = lane >> 5;  // K half of the operands; row quad of the outputs
+    const int j = lane & 31;  // operand index; output column (row of the tile)
+    const int xcd = blockIdx.x & 7;
+    const int64_t s = blockIdx.x >> 3;
+    const int64_t per = (nrc + 7) / 8;
+    if (s / qtiles >= per) return;
+    const int64_t rc = xcd * per + s / qtiles;
+    if (rc >= nrc) return;
+    const int qtile = (int)(s % qtiles);
+    const int64_t r_begin = a.row_begin + rc * chunk_rows;
+    const int64_t r_end = r_begin + chunk_rows < a.row_end ? r_begin + chunk_rows : a.row_end;
+    const int q0 = qtile * QT;
+    const int qbytes = a.kdim;
+    const int ldq = qbytes + 16;
+    const WideLds L = wide_stage<int>(a, smem, QT, ldq, qbytes, q0);
+    const int* bndg_s = static_cast<const int*>(L.bndg);
+    WideQueue wq;
+    wq.row = L.cnt + NW + (int64_t)wave * 2 * qcap;
+    wq.key = reinterpret_cast<float*>(wq.row + qcap);
+    wq.count = L.cnt + wave;
+    wq.hist = L.cnt + NW + (int64_t)NW * 2 * qcap + wave * kFlushSlots;
+    wq.cap = qcap;
+    int* stage = reinterpret_cast<int*>(L.cnt + NW + (int64_t)NW * 2 * qcap + NW * kFlushSlots) + wave * 1024;
+    const float kap = wide_kap(a, L, QT, lane);
+
+    const uint8_t* X = static_cast<const uint8_t*>(a.X);
+    const int nks = qbytes / 32;  // a multiple of D (launcher)
+    constexpr int TR = 32 * RS;
+    const int64_t ntiles = (r_end - r_begin + TR - 1) / TR;
+    if (wave >= ntiles) return;
+    const uint8_t* xbase = X + (r_begin >> 4) * 16 * a.xstride;
+    // K step k of a row: 64-byte chunk k / 2 of its 16-row tile, half k % 2
+    auto koff = [&](int k) -> int64_t { return (int64_t)(k >> 1) * 1024 + (k & 1) * 32; };
+    auto tile_offs = [&](int64_t t, uint32_t (&o)[RS]) {
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            int64_t row = t * TR + 32 * r + j;
+            if (row >= r_end - r_begin) row = r_end - r_begin - 1;
+            const int64_t R = r_begin + row;
+            o[r] = (uint32_t)(((R >> 4) - (r_begin >> 4)) * 16 * a.xstride + (R & 15) * 64) + (uint32_t)(16 * h);
+        }
+    };
+    const uint8_t* qlane = smem + j * ldq + 16 * h;
+
+    i32x4 ring[D][RS];
+    uint32_t cur[RS], nxt[RS];
+    tile_offs(wave, cur);
+#pragma unroll
+    for (int s2 = 0; s2 < D; ++s2) {
+        const uint8_t* xb = xbase + koff(s2);
+#pragma unroll
+        for (int r = 0; r < RS; ++r) ring[s2][r] = *reinterpret_cast<const i32x4*>(xb + cur[r]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    float xnn[RS];
+    auto load_xn = [&](int64_t t) {
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            const int64_t row = r_begin + t * TR + 32 * r + j;
+            xnn[r] = a.xn[row < r_end ? row : r_end - 1];
+        }
+    };
+    load_xn(wave);
+    // query fragments alternate between two register sets: K step c
+    // multiplies qa[c & 1] while the next step's are read (D and the K step
+    // count are even, so the parity carries across turns and wave steps)
+    i32x4 qa[2][QS];
+#pragma unroll
+    for (int q = 0; q < QS; ++q) qa[0][q] = *reinterpret_cast<const i32x4*>(qlane + 32 * q * ldq);
+
+    for (int64_t t = wave; t < ntiles; t += NW) {
+        const int64_t r0 = r_begin + t * TR;
+        tile_offs(t, cur);
+        tile_offs(t + NW < ntiles ? t + NW : t, nxt);
+        float xn[RS];
+        int dofs[RS];
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            xn[r] = xnn[r];
+            dofs[r] = wide_row_offset(a, xn[r], kap);
+        }
+        load_xn(t + NW < ntiles ? t + NW : t);
+        i32x16 acc[QS][RS];
+        auto turn = [&](int c0, auto first) {
+            constexpr bool kFirst = decltype(first)::value;
+            const bool same = c0 + D < nks;  // all slots prefetch from this wave step
+#pragma unroll
+            for (int s2 = 0; s2 < D; ++s2) {
+                const int c = c0 + s2;
+                const int c1 = c + 1 < nks ? c + 1 : 0;
+#pragma unroll
+                for (int q = 0; q < QS; ++q)
+                    qa[(s2 + 1) & 1][q] = *reinterpret_cast<const i32x4*>(qlane + 32 * q * ldq + 32 * c1);
+#pragma unroll
+                for (int q = 0; q < QS; ++q)
+#pragma unroll
+                    for (int r = 0; r < RS; ++r) {
+                        if (kFirst && s2 == 0) {
+#pragma unroll
+                            for (int e = 0; e < 16; ++e) acc[q][r][e] = -dofs[r];
+                        }
+                        W::mma(acc[q][r], qa[s2 & 1][q], ring[s2][r]);
+                    }
+                const int cn = c + D;
+                const int cc = same ? cn : cn - nks;
+                const uint8_t* xb = xbase + koff(cc);
+#pragma unroll
+                for (int r = 0; r < RS; ++r)
+                    ring[s2][r] = *reinterpret_cast<const i32x4*>(xb + (same ? cur[r] : nxt[r]));
+            }
+        };
+        turn(0, std::true_type{});
+        for (int c0 = D; c0 < nks; c0 += D) turn(c0, std::false_type{});
+
+        // prefilter: query tile q may pass in this lane when some element's
+        // max over the lane's rows of dot - D_r reaches its query's bound
+        uint32_t pend = 0;
+#pragma unroll
+        for (int q = 0; q < QS; ++q) {
+            bool maybe = false;
+#pragma unroll
+            for (int e4 = 0; e4 < 4; ++e4) {
+                const i32x4 b = *reinterpret_cast<const i32x4*>(bndg_s + 32 * q + 8 * e4 + 4 * h);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    int mx = acc[q][0][4 * e4 + i];
+#pragma unroll
+                    for (int r = 1; r < RS; ++r) mx = acc[q][r][4 * e4 + i] > mx ? acc[q][r][4 * e4 + i] : mx;
+                    maybe |= mx >= b[i];
+                }
+            }
+            pend |= maybe ? 1u << q : 0u;
+        }
+        if (__ballot(pend != 0u) != 0) {
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                if (__ballot((pend >> q) & 1u) == 0) continue;
+#pragma unroll
+                for (int r = 0; r < RS; ++r) {
+                    // this lane's 16 values of tile (q, r), read back below
+                    // by the same lane only (LDS accesses of a lane stay in
+                    // order)
+#pragma unroll
+                    for (int e4 = 0; e4 < 4; ++e4)
+                        *reinterpret_cast<i32x4*>(stage + (e4 * 64 + lane) * 4) =
+                            (i32x4){acc[q][r][4 * e4], acc[q][r][4 * e4 + 1], acc[q][r][4 * e4 + 2],
+                                    acc[q][r][4 * e4 + 3]};
+                    const int64_t row = r0 + 32 * r + j;
+                    if (((pend >> q) & 1u) && row < r_end) {
+                        const float xr = xn[r];
+                        const int dr = dofs[r];
+#pragma unroll 1
+                        for (int e = 0; e < 16; ++e) {
+                            const int m = 32 * q + 8 * (e >> 2) + 4 * h + (e & 3);
+                            const float th = L.thr[m], qv = L.qn[m], sc = 2.0f * L.qsc[m];
+                            const float dot = (float)(stage[((e >> 2) * 64 + lane) * 4 + (e & 3)] + dr);
+                            const float dist = xr + qv - sc * dot;
+                            if (dist <= th)
+                                wide_append(a, wq, q0 + m, m, (uint32_t)(row - r_begin), (uint32_t)row, dist);
+                        }
+                    }
+                }
+            }
+        }
+        // keep room for a wave step's typical appends
+        if (__builtin_amdgcn_readfirstlane(*wq.count) + 512u > (uint32_t)qcap) wide_flush(a, wq, q0, r_begin, lane);
+    }
+    wide_flush(a, wq, q0, r_begin, lane);
+}
+
+template <int RS, int D, int NW>
+hipError_t launch_i8x32_t(const ScanArgs& a, hipStream_t st) {
+    static const char* const name = [] {
+        static char buf[80];
+        snprintf(buf, sizeof(buf), "scan_i8x32_kernel<sq8,RS=%d,D=%d,NW=%d>", RS, D, NW);
+        return buf;
+    }();
+    t_variant = name;
+    constexpr int QT = 128;
+    const int qbytes = a.kdim;
+    if (!a.tiled || a.prefilter != 3 || qbytes % (32 * D) != 0) return hipErrorInvalidValue;
+    const size_t lds = (size_t)QT * (qbytes + 16) + 5 * QT * sizeof(float) + NW * sizeof(uint32_t);
+    const size_t hist_bytes = (size_t)NW * kFlushSlots * sizeof(uint32_t);
+    const size_t stage_bytes = (size_t)NW * 4096;
+    if (lds + hist_bytes + stage_bytes + (size_t)NW * 1024 * 8 > 160 * 1024) return hipErrorInvalidValue;
+    static const bool attr = [] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_i8x32_kernel<RS, D, NW>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        return true;
+    }();
+    (void)attr;
+    const int ncu = device_cu_count();
+    const int qtiles = (a.B + QT - 1) / QT;
+    const int64_t rows = a.row_end - a.row_begin;
+    constexpr int64_t step = 32 * RS * NW;
+    static const int64_t chunk_env = [] {
+        const char* e = getenv("ANNMI_WIDE_CHUNK_MAX");
+        const long v = e ? atol(e) : 0;
+        return v >= 2048 ? (int64_t)v : (int64_t)32768;
+    }();
+    const int64_t chunk_max = a.chunk_max >= 2048 ? a.chunk_max : chunk_env;
+    // a few workgroups a CU over the launch, no more than the query tiles
+    const int64_t wg_per_cu = qtiles < 8 ? qtiles : 8;
+    int64_t chunk = rows * qtiles / ((int64_t)ncu * wg_per_cu);
+    chunk = chunk < step ? step : (chunk > chunk_max ? chunk_max : chunk);
+    chunk = (chunk + step - 1) / step * step;
+    const int64_t nrc = (rows + chunk - 1) / chunk;
+    const int64_t per = (nrc + 7) / 8;
+    int qcap = (int)((160 * 1024 - lds - hist_bytes - stage_bytes) / (NW * 8)) / 64 * 64;
+    if (qcap > 4096) qcap = 4096;
+    if (const char* e = getenv("ANNMI_WIDE_QCAP")) {  // lower cap: tuning and tests
+        const int v = atoi(e);
+        if (v >= 0 && v < qcap) qcap = v / 64 * 64;
+    }
+    if (qcap < 1024) qcap = 0;
+    ANNMI_LAUNCH((scan_i8x32_kernel<RS, D, NW>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
+                 lds + (size_t)NW * qcap * 8 + hist_bytes + stage_bytes, st, a, qtiles, chunk, nrc, qcap);
+    return hipGetLastError();
+}
+
 // LDS of a wide configuration's query tile: 16*QS queries of qrow bytes plus
 // five per-query floats (the queues take what is left of the 160 KB).
 template <class P>
@@ -1382,10 +1637,30 @@ struct W32Of<PolBF16> {
     typedef W32BF16 T;
 };
 
+// 32-byte K steps of a row of the 32 x 32 kernel's element type
+template <class W>
+int nks_of(const ScanArgs& a) {
+    return a.kdim * W::kElemBytes / 32;
+}
+
 template <class P>
 hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
     typedef typename W32Of<P>::T W;
     if constexpr (!std::is_void<W>::value) {
+        if constexpr (std::is_same<P, PolSQ8>::value) {
+            if ((a.wide_cfg == kWideCfg32R4W4 || a.wide_cfg == kWideCfg32R2W4) && a.tiled && a.prefilter == 3) {
+                const bool d4 = nks_of<W>(a) % 4 == 0;
+                if (a.wide_cfg == kWideCfg32R4W4)
+                    return d4 ? launch_i8x32_t<1, 4, 8>(a, st) : launch_i8x32_t<1, 2, 8>(a, st);
+                return d4 ? launch_i8x32_t<2, 4, 8>(a, st) : launch_i8x32_t<2, 2, 8>(a, st);
+            }
+        }
+        if (a.wide_cfg == kWideCfg32R4W4)
+            return nks_of<W>(a) % 4 == 0 ? launch_wide32_t<W, 4, 4, 4, 4, true>(a, st)
+                                         : launch_wide32_t<W, 4, 4, 4, 2, true>(a, st);
+        if (a.wide_cfg == kWideCfg32R2W4)
+            return nks_of<W>(a) % 4 == 0 ? launch_wide32_t<W, 4, 2, 4, 4, true>(a, st)
+                                         : launch_wide32_t<W, 4, 2, 4, 2, true>(a, st);
         if (a.wide_cfg == kWideCfg32R1 || a.wide_cfg == kWideCfg32R2) {
             // ring depth: the deepest of 8, 6, 4, 2 K steps dividing the row's
             // (rows are whole 64-byte chunks: an even count); two row tiles a
@@ -1478,7 +1753,8 @@ hipError_t launch_scan(const ScanArgs& a, hipStream_t st, const char** variant) 
 bool wide_cfg_fits(int kind, int kdim, int cfg) {
     const int qs = cfg == kWideCfgQ4R4W8 ? 4 : 8;
     if (cfg <= kWideCfgDefault || cfg >= kWideCfgCount) return false;
-    if (cfg == kWideCfg32R1 || cfg == kWideCfg32R2) {  // 128 queries, whole 64-byte row chunks
+    if (cfg == kWideCfg32R1 || cfg == kWideCfg32R2 || cfg == kWideCfg32R4W4 ||
+        cfg == kWideCfg32R2W4) {  // 128 queries, whole 64-byte row chunks
         if (kind != SCAN_SQ8 && kind != SCAN_BF16) return false;
         const int qbytes = kind == SCAN_SQ8 ? kdim : 2 * kdim;
         return qbytes % 64 == 0 && (size_t)128 * (qbytes + 16 + 20) + 8 * 1024 <= 160 * 1024;

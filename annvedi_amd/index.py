@@ -121,6 +121,20 @@ class Index:
             raise RuntimeError(_lib.last_error())
         return ids[:, :got], dd[:, :got]
 
+    def search_gpu_filter(self, q, k: int, r: int = 150):
+        """gf_query on this index (built as mode sq8, sq_variant 1, as
+        gf_create builds it): the reference GPU contestant's ids for one
+        query (competitors/gpu_filter/src/gpu_filter.cu:142-176)."""
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.float32)).ravel()
+        if q.shape[0] != self.d:
+            raise RuntimeError(f"query must be a 1-D vector of length {self.d}")
+        out = np.empty(int(k), dtype=np.int64)
+        got = self._L.gf_query(self._h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(k), int(r),
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+        if got < 0:
+            raise RuntimeError((self._L.gf_last_error() or b"").decode(errors="replace"))
+        return out[:got]
+
     def search_linear_index(self, Q, k: int):
         """The reference's LinearIndex.query semantics bit for bit (ids and
         the fp32 values it ranks, early-abandon partial sums included)."""

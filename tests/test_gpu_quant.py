@@ -160,3 +160,20 @@ def test_tiled_scan_on_unaligned_ranges_and_batches(B):
     for i in range(min(B, 4)):
         assert len(set(ids[i]) & set(ref_ids[i])) >= 97
         assert len(set(fids[i]) & set(ref_ids[i])) >= 99
+
+
+def test_plugin_gpu_backend_is_the_reference_contestant():
+    """backend: gpu (competitors/hybrid/algorithm.py:54-95) answers with the
+    gf_query chain: ids equal the reference's, r from the query params
+    (default 150), n_distances += r per query."""
+    from annvedi_amd import Algorithm
+
+    base, Q = data.workload("exact-1m-384", n=15000, n_queries=4)
+    a = Algorithm()
+    a.fit(base, backend="gpu", M=16, ef_construction=100, mode="sq8pd")
+    np.testing.assert_array_equal(a.query(Q[0], 100), O.gf_query_reference(base, Q[0], 100, 150)[0])
+    np.testing.assert_array_equal(a.query(Q[1], 100, r=500), O.gf_query_reference(base, Q[1], 100, 500)[0])
+    ids, _ = a.batch_query(Q[2:], 10, r=200)
+    for i in range(2):
+        np.testing.assert_array_equal(ids[i], O.gf_query_reference(base, Q[2 + i], 10, 200)[0])
+    assert a.get_n_distances() == 150 + 500 + 2 * 200
