@@ -302,6 +302,34 @@ def copy_calibration(gib: float):
     print(json.dumps({"copy_bytes": nbytes}))
 
 
+def bench_served(idx, data, gen, d, C, sigma, cid, k, R, closed_qps, n_req=64, workers=2, load=0.8):
+    """Open-loop serving of the timed index through serving.Server: n_req
+    requests, sizes log-uniform in [1, 4096], Poisson arrivals offered at
+    `load` x the closed-loop throughput, device-resident queries, an HBM
+    budget of the index plus `workers` slots of 4096 queries.  Latency is
+    per request, from its arrival to its answers."""
+    import torch
+
+    from annvedi_amd.serving import SLOT_BYTES_PER_QUERY, Server, loguniform_sizes, run_open_loop
+
+    sizes = loguniform_sizes(n_req, 1, 4096, seed=4002)
+    warm = np.array([4096, 4096, 17, 3])
+    Qs = data.mixture_torch(int(sizes.sum() + warm.sum()), d, C, sigma, data.SEED_QUERIES + cid + 10, **gen)
+    budget = idx.nbytes + workers * 4096 * SLOT_BYTES_PER_QUERY
+    offered = load * closed_qps
+    with Server(idx, workers=workers, max_batch=4096, hbm_budget=budget, rerank=R) as srv:
+        run_open_loop(srv, Qs[: int(warm.sum())], warm, qps=offered, k=k, seed=4003)  # workspace, warm kernels
+        res = run_open_loop(srv, Qs[int(warm.sum()):], sizes, qps=offered, k=k, seed=4001)
+        peak = srv.peak_reserved
+    torch.cuda.synchronize()
+    return {"offered_qps": round(offered, 1), "qps": round(res["qps"], 1),
+            "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
+            "requests": int(len(sizes)), "queries": int(res["queries"]), "workers": workers,
+            "request_sizes": "log-uniform [1, 4096]", "arrivals": "Poisson",
+            "latency": "per request, arrival to completion", "hbm_budget_bytes": int(budget),
+            "peak_workspace_bytes": int(peak)}
+
+
 def measured_traffic(workload: str, mode: str, kernel: str, alg_bytes_per_launch: float):
     """HBM bytes per launch of the dominant kernel from the counter runs
     summarised in profiles/traffic_<workload>_<mode>.json: FETCH_SIZE and
@@ -341,6 +369,7 @@ def main():
     ap.add_argument("--rerank", type=int, default=0, help="fixed R (0: calibrate to recall >= 0.99)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain fp32 path")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served (open-loop) leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in scan-kernel shapes")
     ap.add_argument("--fixed-batch", type=int, default=0,
                     help="every step has this many queries instead of the log-uniform mix (e.g. 1: the "
@@ -459,6 +488,14 @@ def main():
                      "scan_ms": round(pprof["scan_ms"], 2)}
             log(f"[bench] plain fp32: {plain}")
 
+    # served leg: the same index behind serving.Server, requests arriving as
+    # a Poisson process with sizes log-uniform in [1, 4096], latency from
+    # arrival to completion of each request
+    served = None
+    if not args.no_serve and mode != "exact" and args.fixed_batch == 0:
+        served = bench_served(idx, data, gen, d, C, sigma, cid, k, R, qps)
+        log(f"[bench] served: {served}")
+
     # roofline of the dominant kernel: the scan instantiation with the most
     # event-timed time in the timed region (in-library HIP events on the
     # stream the kernels run on)
@@ -535,6 +572,7 @@ def main():
         "dist_rel_err_max_vs_fp64": dist_err,
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "served": served,
         "plain_fp32": plain,
         "speedup_vs_plain_fp32": round(qps / plain["qps"], 3) if plain else None,
         "roofline": roof,

@@ -55,3 +55,37 @@ def test_repeated_shapes_on_cloned_handles_across_threads(mode):
         a_ids, a_dd = idx.search(Q[off:off + b], 100, rerank=200, nprobe=8)
         np.testing.assert_array_equal(ids, a_ids)
         off += b
+
+
+def test_device_requests_under_an_hbm_budget():
+    """Requests as device tensors, served under a budget of the index plus
+    two slots: budgets below the index or below one slot are refused, the
+    workers are capped at the slots that fit, the bytes held stay within the
+    budget and every answer equals the one computed alone."""
+    import torch
+
+    from annvedi_amd.serving import SLOT_BYTES_PER_QUERY
+
+    print("[case] served device requests under a budget", flush=True)
+    base, Q = data.workload("exact-1m-384", n=100_000, n_queries=1000)
+    idx = Index(base, mode="sq8", rerank=200)
+    slot = 64 * SLOT_BYTES_PER_QUERY
+    assert idx.nbytes > 0
+    with pytest.raises(ValueError, match="below the index"):
+        Server(idx, workers=2, max_batch=64, hbm_budget=idx.nbytes - 1)
+    with pytest.raises(ValueError, match="one worker slot"):
+        Server(idx, workers=2, max_batch=64, hbm_budget=idx.nbytes + slot - 1)
+    dQ = torch.from_numpy(Q).cuda()
+    sizes = lognormal_sizes(100, seed=5, hi=64)
+    sizes = sizes[np.cumsum(sizes) <= Q.shape[0]]
+    with Server(idx, workers=3, max_batch=64, hbm_budget=idx.nbytes + 2 * slot, rerank=200) as srv:
+        assert len(srv.handles) == 2
+        res = run_open_loop(srv, dQ, sizes, qps=20000, k=100)
+        assert 0 < srv.peak_reserved <= 2 * slot
+    off = 0
+    for b, (ids, dd) in zip(sizes, res["answers"]):
+        assert ids.is_cuda
+        a_ids, a_dd = idx.search(Q[off:off + b], 100, rerank=200)
+        np.testing.assert_array_equal(ids.cpu().numpy(), a_ids)
+        np.testing.assert_array_equal(dd.cpu().numpy(), a_dd)
+        off += b

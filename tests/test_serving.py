@@ -83,3 +83,53 @@ def test_open_loop_poisson_accounting():
         np.testing.assert_array_equal(ids[:, 0], np.arange(off, off + b) * 1000)
         off += b
     assert 0 < res["p50_ms"] <= res["p99_ms"]
+
+
+class SizedIndex:
+    d = 4
+    nbytes = 10_000
+
+
+def test_budget_below_the_index_or_one_slot_is_refused():
+    S = 8 * SLOT_BYTES_PER_QUERY
+    with pytest.raises(ValueError, match="below the index"):
+        Server(SizedIndex(), workers=2, max_batch=8, hbm_budget=9_999, search=fake_search_factory())
+    with pytest.raises(ValueError, match="one worker slot"):
+        Server(SizedIndex(), workers=2, max_batch=8, hbm_budget=10_000 + S - 1, search=fake_search_factory())
+    srv = Server(SizedIndex(), workers=5, max_batch=8, hbm_budget=10_000 + 2 * S, search=fake_search_factory())
+    assert len(srv.handles) == 2  # the slots that fit next to the index
+    with pytest.raises(ValueError):
+        srv.set_budget(10_000 + S - 1)
+    srv.close()
+
+
+def test_live_budget_limits_the_requests_in_flight():
+    """Two workers, then a budget of one slot: requests of 6 queries (two do
+    not share an 8-query slot) run one at a time, and the bytes held never
+    exceed the budget left by the index."""
+    S = 8 * SLOT_BYTES_PER_QUERY
+    active, peak, lock = [0], [0], threading.Lock()
+
+    def search(h, Q, k):
+        with lock:
+            active[0] += 1
+            peak[0] = max(peak[0], active[0])
+        time.sleep(0.02)
+        with lock:
+            active[0] -= 1
+        return np.zeros((Q.shape[0], k), np.int64), np.zeros((Q.shape[0], k), np.float32)
+
+    srv = Server(SizedIndex(), workers=2, max_batch=8, hbm_budget=10_000 + 2 * S, search=search)
+    futs = [srv.submit(np.zeros((6, 4), np.float32), 3) for _ in range(4)]
+    for f in futs:
+        f.result(timeout=10)
+    assert peak[0] == 2  # two slots: two requests on the GPU at once
+    srv.set_budget(10_000 + S)
+    peak[0] = 0
+    srv.peak_reserved = 0
+    futs = [srv.submit(np.zeros((6, 4), np.float32), 3) for _ in range(6)]
+    for f in futs:
+        f.result(timeout=10)
+    srv.close()
+    assert peak[0] == 1
+    assert srv.peak_reserved <= S
