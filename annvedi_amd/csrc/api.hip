@@ -158,6 +158,7 @@ struct Handle {
     float* sq_scale = nullptr;
     float* sq_offset = nullptr;
     hipStream_t st = nullptr;
+    bool own_stream = true;  // false: the parent IVF handle's stream (centroid index)
     int64_t bytes = 0;
     // workspace
     int ws_B = 0;
@@ -209,20 +210,21 @@ struct Handle {
     std::vector<int64_t> list_off;  // host, nlist + 1
     float* centroids = nullptr;     // [nlist x kdim32]
     struct Handle* cent = nullptr;  // exact index over the centroids (probe step)
-    int64_t* d_probe = nullptr;     // [ws_B x nprobe_cap]
+    int64_t* d_probe = nullptr;     // [ws_B x probe_cap] probed list ids
     float* d_probe_d = nullptr;
     int probe_cap = 0;
-    IvfItem* d_items = nullptr;
-    int items_cap = 0;
-    // host-side IVF step buffers, reused across steps
-    std::vector<std::vector<uint32_t>> ivf_qlist;
-    std::vector<IvfItem> ivf_items_host;
-    std::vector<uint32_t> ivf_gq_host;
-    uint32_t* d_gq = nullptr;
-    int64_t gq_cap = 0;
-    uint32_t* d_next = nullptr;
-    int next_cap = 0;
-    std::vector<int64_t> last_scanned;
+    // device plan of an IVF step (ivf.h IvfPlanArgs); sized for ws_B queries
+    int64_t* d_list_off = nullptr;  // [nlist + 1] (index array, shared by clones)
+    int64_t max_list_len = 0;
+    uint32_t* d_qcount = nullptr;   // [nlist]
+    uint32_t* d_qoff = nullptr;     // [nlist]
+    uint32_t* d_qfill = nullptr;    // [nlist]
+    uint8_t* d_every = nullptr;     // [plan_B]
+    uint32_t* d_gq = nullptr;       // [plan_B x nlist]
+    uint32_t* d_plan = nullptr;     // [kIvfMaxPass x (nlist + 1)]
+    uint32_t* d_nitems = nullptr;   // [kIvfMaxPass]
+    uint32_t* d_next = nullptr;     // [kIvfMaxPass]
+    int plan_B = 0;
     // a clone shares every index array with its source and owns only its
     // stream and workspace
     bool is_clone = false;
@@ -242,7 +244,7 @@ struct Handle {
     // replayed from then on.  Workspace pointers are baked in, so the cache
     // is dropped whenever the workspace is reallocated.
     struct StepGraph {
-        int B, k, rsel;
+        int B, k, rsel, nprobe;
         const void* ids;
         const void* dist;
         hipGraphExec_t exec;  // null: seen once, not captured yet
@@ -330,6 +332,7 @@ void free_handle(Handle* h) {
         dfree(h->sq_offset);
         dfree(h->perm);
         dfree(h->centroids);
+        dfree(h->d_list_off);
     }
     dfree(h->scratch_keys);
     dfree(h->rb_keys);
@@ -352,12 +355,17 @@ void free_handle(Handle* h) {
     dfree(h->li_flag);
     dfree(h->d_probe);
     dfree(h->d_probe_d);
-    dfree(h->d_items);
+    dfree(h->d_qcount);
+    dfree(h->d_qoff);
+    dfree(h->d_qfill);
+    dfree(h->d_every);
     dfree(h->d_gq);
+    dfree(h->d_plan);
+    dfree(h->d_nitems);
     dfree(h->d_next);
     if (h->cent) free_handle(h->cent);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
-    if (h->st) (void)hipStreamDestroy(h->st);
+    if (h->st && h->own_stream) (void)hipStreamDestroy(h->st);
     delete h;
 }
 
@@ -572,12 +580,17 @@ bool run_robust(Handle* h, const std::vector<int32_t>& qs, int k, int64_t* ids_o
 // near-tie window, an fp64 run longer than 32) are answered again by the
 // exact all-rows path (flag bit 3 marks them in annmi_last_batch_stats).
 bool complete_step(Handle* h, int B, int k, int64_t* ids_out, float* d_out, bool exact, Profiler& pr, int ev_f0,
-                   int ev_f1) {
+                   int ev_f1, const uint32_t* probe_flags) {
     hipStream_t st = h->st;
-    std::vector<uint32_t> flags(B), nrr(B);
+    std::vector<uint32_t> flags(B), nrr(B), pflags;
     HIP_OR(hipMemcpyAsync(flags.data(), h->ovf, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
     HIP_OR(hipMemcpyAsync(nrr.data(), h->nrr, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
+    if (probe_flags) {  // the IVF probe's own flags: a query whose probe could not be bounded
+        pflags.resize(B);
+        HIP_OR(hipMemcpyAsync(pflags.data(), probe_flags, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
+    }
     HIP_OR(hipStreamSynchronize(st), false);
+    for (size_t i = 0; i < pflags.size(); ++i) flags[i] |= pflags[i] & 7u;
     if (h->prof && ev_f1 >= 0) {
         auto& scan_ev = *pr.scan_ev;
         auto& sel_ev = *pr.sel_ev;
@@ -644,16 +657,12 @@ bool complete_step(Handle* h, int B, int k, int64_t* ids_out, float* d_out, bool
     return true;
 }
 
-bool finish_step(Handle* h, int B, int k, int kk, int rsel, int64_t* ids_out, float* d_out, const CandBuf& cb,
-                 bool exact, Profiler& pr) {
-    (void)kk;
-    int ev_f0 = -1, ev_f1 = -1;
-    if (!launch_final(h, B, k, rsel, ids_out, d_out, cb, pr, ev_f0, ev_f1)) return false;
-    return complete_step(h, B, k, ids_out, d_out, exact, pr, ev_f0, ev_f1);
-}
 
 
-bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, float* d_out);
+bool ensure_ivf_ws(Handle* h, int nprobe);
+bool enqueue_ivf(Handle* h, int B, int k, int rsel, int nprobe, int64_t* ids_out, float* d_out, Profiler& pr,
+                 std::vector<std::pair<int, int>>& scan_ev, std::vector<std::pair<int, int>>& sel_ev,
+                 std::vector<double>& scan_rows, std::vector<std::string>& scan_names, int& ev_f0, int& ev_f1);
 int32_t batch_impl(Handle* h, const float* Q, int32_t B, int32_t k, int32_t R, int64_t* ids, float* dists,
                    bool device, int32_t nprobe);
 Handle* create_impl(const float* base, int64_t n, int32_t d, const annmi_params* params, int64_t src_stride,
@@ -734,17 +743,28 @@ bool graphs_enabled() {
 constexpr size_t kMaxGraphs = 32;
 
 // One step over <= ws_B queries already in h->dQ32; results in d_ids/d_dist.
-// Flat steps of a shape seen before replay a captured graph (no per-kernel
-// launch cost); the first step of a shape runs eagerly, the second captures.
+// Steps of a shape seen before replay a captured graph (no per-kernel launch
+// cost); the first step of a shape runs eagerly, the second captures.  IVF
+// steps are planned on the device and captured the same way.
 bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, float* d_out) {
-    if (h->ivf) return run_step_ivf(h, B, k, R, nprobe, ids_out, d_out);
-    const bool exact = h->kind == SCAN_F32;
+    const bool exact = h->kind == SCAN_F32 && !h->ivf;
     const int kk = (int)std::min<int64_t>(k, h->n);
     int rsel = exact ? kk : std::max(kk, R);
     rsel = (int)std::min<int64_t>(rsel, h->n);
     if (!exact && rsel > 2048) {
         set_err("rerank R=%d exceeds the 2048-row re-score limit", rsel);
         return false;
+    }
+    if (h->ivf) {
+        if (nprobe <= 0) nprobe = h->p.nprobe > 0 ? h->p.nprobe : 8;
+        nprobe = std::min(nprobe, h->nlist);
+        if (nprobe > 2048) {
+            set_err("nprobe=%d exceeds 2048", nprobe);
+            return false;
+        }
+        if (!ensure_ivf_ws(h, nprobe)) return false;
+    } else {
+        nprobe = 0;
     }
     hipStream_t st = h->st;
     std::vector<std::pair<int, int>> scan_ev, sel_ev;  // (start, end) event indices
@@ -753,10 +773,18 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
     int ev_used = 0;
     Profiler pr{h, ev_used, &scan_ev, &sel_ev, &scan_rows, &scan_names, -1, -1};
     int ev_f0 = -1, ev_f1 = -1;
+    auto enqueue = [&]() {
+        return h->ivf ? enqueue_ivf(h, B, k, rsel, nprobe, ids_out, d_out, pr, scan_ev, sel_ev, scan_rows, scan_names,
+                                    ev_f0, ev_f1)
+                      : enqueue_flat(h, B, k, rsel, ids_out, d_out, pr, scan_ev, sel_ev, scan_rows, scan_names, ev_f0,
+                                     ev_f1);
+    };
+    const uint32_t* probe_flags = h->ivf ? h->cent->ovf : nullptr;
     Handle::StepGraph* sg = nullptr;
     if (graphs_enabled() && !h->prof && !h->tuning && !h->shared.load()) {
         for (auto& g : h->graphs)
-            if (g.B == B && g.k == k && g.rsel == rsel && g.ids == ids_out && g.dist == d_out) sg = &g;
+            if (g.B == B && g.k == k && g.rsel == rsel && g.nprobe == nprobe && g.ids == ids_out && g.dist == d_out)
+                sg = &g;
         if (!sg) {
             if (h->graphs.size() >= kMaxGraphs) {  // evict the least recently used
                 size_t lru = 0;
@@ -765,14 +793,13 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
                 if (h->graphs[lru].exec) (void)hipGraphExecDestroy(h->graphs[lru].exec);
                 h->graphs.erase(h->graphs.begin() + (long)lru);
             }
-            h->graphs.push_back(Handle::StepGraph{B, k, rsel, ids_out, d_out, nullptr, false, ++h->graph_clock});
+            h->graphs.push_back(Handle::StepGraph{B, k, rsel, nprobe, ids_out, d_out, nullptr, false, ++h->graph_clock});
         } else {
             sg->last_use = ++h->graph_clock;
             if (!sg->exec && !sg->failed) {
                 // second sighting of this shape: capture the step, then replay it
                 HIP_OR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), false);
-                const bool ok = enqueue_flat(h, B, k, rsel, ids_out, d_out, pr, scan_ev, sel_ev, scan_rows, scan_names,
-                                             ev_f0, ev_f1);
+                const bool ok = enqueue();
                 hipGraph_t graph = nullptr;
                 const hipError_t ec = hipStreamEndCapture(st, &graph);
                 if (!ok) {
@@ -794,13 +821,12 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
             }
             if (sg->exec) {
                 HIP_OR(hipGraphLaunch(sg->exec, st), false);
-                return complete_step(h, B, k, ids_out, d_out, exact, pr, -1, -1);
+                return complete_step(h, B, k, ids_out, d_out, exact, pr, -1, -1, probe_flags);
             }
         }
     }
-    if (!enqueue_flat(h, B, k, rsel, ids_out, d_out, pr, scan_ev, sel_ev, scan_rows, scan_names, ev_f0, ev_f1))
-        return false;
-    return complete_step(h, B, k, ids_out, d_out, exact, pr, ev_f0, ev_f1);
+    if (!enqueue()) return false;
+    return complete_step(h, B, k, ids_out, d_out, exact, pr, ev_f0, ev_f1, probe_flags);
 }
 
 bool ivf_build(Handle* h);
@@ -838,6 +864,35 @@ bool xn_bounds(Handle* h) {
             if (v >= 1 && v <= 3) h->prefilter = v;
         }
     }
+    return true;
+}
+
+// A handle's centroid index runs on its parent's stream (its steps are part
+// of the parent's IVF step and are captured with it).
+void share_stream(Handle* child, Handle* parent) {
+    if (child->st && child->own_stream) (void)hipStreamDestroy(child->st);
+    child->st = parent->st;
+    child->own_stream = false;
+}
+
+// IVF index arrays beside the lists: the exact index over the centroids (the
+// probe step, lsh_index_optimized.cpp:325-333) and the list offsets on the
+// device for the step plan.
+bool ivf_attach(Handle* h) {
+    annmi_params cp;
+    annmi_default_params(&cp, ANNMI_EXACT_F32);
+    cp.base_on_device = 1;
+    cp.max_batch = h->p.max_batch;
+    h->cent = create_impl(h->centroids, h->nlist, h->d, &cp, h->kdim32);
+    if (!h->cent) return false;
+    share_stream(h->cent, h);
+    h->max_list_len = 0;
+    for (int l = 0; l < h->nlist; ++l)
+        h->max_list_len = std::max<int64_t>(h->max_list_len, h->list_off[(size_t)l + 1] - h->list_off[(size_t)l]);
+    HIP_OR(hipMalloc(&h->d_list_off, h->list_off.size() * 8), false);
+    HIP_OR(hipMemcpyAsync(h->d_list_off, h->list_off.data(), h->list_off.size() * 8, hipMemcpyHostToDevice, h->st),
+           false);
+    HIP_OR(hipStreamSynchronize(h->st), false);
     return true;
 }
 
@@ -917,15 +972,7 @@ bool setup_mode(Handle* h) {
                          h->p.sq_variant, st, h->ivf ? h->perm : nullptr, h->tiled ? tile_cb_of(h->kind) : 0),
            false);
     HIP_OR(hipStreamSynchronize(st), false);
-    if (h->ivf) {
-        // exact index over the centroids: the probe step (lsh_index_optimized.cpp:325-333)
-        annmi_params cp;
-        annmi_default_params(&cp, ANNMI_EXACT_F32);
-        cp.base_on_device = 1;
-        cp.max_batch = h->p.max_batch;
-        h->cent = create_impl(h->centroids, h->nlist, d, &cp, h->kdim32);
-        if (!h->cent) return false;
-    }
+    if (h->ivf && !ivf_attach(h)) return false;
     return true;
 }
 
@@ -1181,14 +1228,7 @@ bool load_mode(Handle* h, const uint8_t* blob) {
             return false;
     }
     HIP_OR(hipStreamSynchronize(st), false);
-    if (h->ivf) {
-        annmi_params cp;
-        annmi_default_params(&cp, ANNMI_EXACT_F32);
-        cp.base_on_device = 1;
-        cp.max_batch = h->p.max_batch;
-        h->cent = create_impl(h->centroids, h->nlist, h->d, &cp, h->kdim32);
-        if (!h->cent) return false;
-    }
+    if (h->ivf && !ivf_attach(h)) return false;
     return true;
 }
 
@@ -1329,6 +1369,8 @@ Handle* clone_impl(Handle* src) {
     h->nlist = src->nlist;
     h->perm = src->perm;
     h->list_off = src->list_off;
+    h->d_list_off = src->d_list_off;
+    h->max_list_len = src->max_list_len;
     h->centroids = src->centroids;
     h->is_clone = true;
     h->shared = true;
@@ -1345,6 +1387,7 @@ Handle* clone_impl(Handle* src) {
             free_handle(h);
             return nullptr;
         }
+        share_stream(h->cent, h);
     }
     return h;
 }
@@ -1475,167 +1518,137 @@ bool ivf_build(Handle* h) {
     return true;
 }
 
-// IVF step: probe (exact search over the centroids), list scan of the
-// probed lists in two passes (a sample of every probed list with an open
-// filter, then the rest with the rsel-th smallest key as filter), finalize
-// (fp32 re-score of rsel candidates from the original rows, original ids).
-bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, float* d_out) {
-    hipStream_t st = h->st;
-    const int kk = (int)std::min<int64_t>(k, h->n);
-    int rsel = std::max(kk, R);  // refine_r, at least k (:472-474)
-    rsel = (int)std::min<int64_t>(rsel, h->n);
-    if (rsel > 2048) {
-        set_err("rerank R=%d exceeds the 2048-row re-score limit", rsel);
-        return false;
-    }
-    if (nprobe <= 0) nprobe = h->p.nprobe > 0 ? h->p.nprobe : 8;
-    nprobe = std::min(nprobe, h->nlist);
-    std::vector<std::pair<int, int>> scan_ev, sel_ev;
-    std::vector<double> scan_rows;
-    std::vector<std::string> scan_names;
-    int ev_used = 0;
-    Profiler pr{h, ev_used, &scan_ev, &sel_ev, &scan_rows, &scan_names, -1, -1};
-    pr.ev_prep0 = pr.ev();
-    HIP_OR(launch_prep_queries(SCAN_SQ8, h->dQ32, B, h->d, h->kdim, h->dQs, h->qn, h->qscale, h->slack, h->maxnorm,
-                               1.f, h->sq_scale, h->sq_offset, st),
-           false);
-    HIP_OR(hipStreamSynchronize(st), false);
-    // probe lists: nprobe nearest centroids, exact order, lower id on ties
-    if (h->probe_cap < h->ws_B * nprobe) {
+// IVF step, enqueued with no host round trip (so it can be captured):
+//   prep_queries             SQ8 operands; resets counts, flags, thresholds
+//   probe                    exact top-nprobe over the centroids (the
+//                            centroid index's flat step on this stream)
+//   plan                     queries per list (every list for a query whose
+//                            probes hold fewer than k rows), item prefixes
+//   list scan per pass       pass 0: the first S rows of every probed list
+//                            with an open filter; pass p: rows [S 8^(p-1),
+//                            S 8^p) against the rsel-th smallest key so far
+//   finalize                 fp32 re-score of rsel candidates, original ids
+constexpr int kIvfMaxPass = 16;
+constexpr int kIvfChunk = 256;  // rows of a list per work item
+
+// First-pass rows of each probed list: enough for ~4 rsel candidates over
+// the probes, within half the candidate list.
+int64_t ivf_first_pass(const Handle* h, int rsel, int nprobe) {
+    int64_t S = std::max<int64_t>(256, (4 * (int64_t)rsel + nprobe - 1) / nprobe);
+    return std::max<int64_t>(16, std::min<int64_t>(S, h->cap / (2 * (int64_t)nprobe)));
+}
+
+int ivf_passes(const Handle* h, int64_t S) {
+    int np = 1;
+    int64_t lo, hi;
+    ivf_pass_bounds(S, 0, lo, hi);
+    while (hi < h->max_list_len && np < kIvfMaxPass) ivf_pass_bounds(S, np++, lo, hi);
+    return np;
+}
+
+// Buffers of the IVF step for ws_B queries probing up to nprobe lists, and
+// the centroid index's workspace (all before any capture).
+bool ensure_ivf_ws(Handle* h, int nprobe) {
+    const int B = h->ws_B;
+    // captured steps hold these buffers' addresses: any reallocation drops them
+    if (h->probe_cap < B * nprobe || h->plan_B < B || !h->d_qcount || h->cent->ws_B < B) drop_graphs(h);
+    if (h->probe_cap < B * nprobe) {
         dfree(h->d_probe);
         dfree(h->d_probe_d);
         h->d_probe = nullptr;
         h->d_probe_d = nullptr;
-        h->probe_cap = h->ws_B * nprobe;
+        h->probe_cap = B * nprobe;
         HIP_OR(hipMalloc(&h->d_probe, (size_t)h->probe_cap * 8), false);
         HIP_OR(hipMalloc(&h->d_probe_d, (size_t)h->probe_cap * 4), false);
     }
-    if (batch_impl(h->cent, h->dQ32, B, nprobe, 0, h->d_probe, h->d_probe_d, true, 0) < 0) return false;
-    std::vector<int64_t> probe((size_t)B * nprobe);
-    // on the handle's stream: a legacy-stream copy is refused while another
-    // thread captures a step graph
-    HIP_OR(hipMemcpyAsync(probe.data(), h->d_probe, probe.size() * 8, hipMemcpyDeviceToHost, st), false);
-    HIP_OR(hipStreamSynchronize(st), false);
-    // work items: (list chunk x group of <= 16 queries probing that list)
-    // per-list query lists, kept by the handle across steps (cleared, not
-    // reallocated)
-    auto& qlist = h->ivf_qlist;
-    if ((int)qlist.size() != h->nlist)
-        qlist.assign((size_t)h->nlist, {});
-    else
-        for (auto& v : qlist) v.clear();
-    // A query whose probed lists hold fewer than k rows scans every list
-    // instead (lsh_index_optimized.cpp:452-457 falls back to all points), so
-    // an answer never holds fewer than min(k, n) rows.
-    std::vector<int64_t> scanned((size_t)B, 0);
-    for (int q = 0; q < B; ++q) {
-        for (int j = 0; j < nprobe; ++j) {
-            const int64_t l = probe[(size_t)q * nprobe + j];
-            if (l >= 0) scanned[(size_t)q] += h->list_off[(size_t)l + 1] - h->list_off[(size_t)l];
-        }
-        if (scanned[(size_t)q] < kk) {
-            for (int l = 0; l < h->nlist; ++l) qlist[(size_t)l].push_back((uint32_t)q);
-            scanned[(size_t)q] = h->n;
-            continue;
-        }
-        for (int j = 0; j < nprobe; ++j) {
-            const int64_t l = probe[(size_t)q * nprobe + j];
-            if (l >= 0) qlist[(size_t)l].push_back((uint32_t)q);
-        }
-    }
-    // geometric passes over every probed list: pass 0 takes the first S rows
-    // of each list with an open filter, pass p the rows [S*8^(p-1), S*8^p);
-    // before each later pass the filter becomes the rsel-th smallest key seen
-    // so far, so each pass adds about rsel * 7 candidates whatever the list
-    // lengths (hub lists of millions of rows included).
-    int64_t S = std::max<int64_t>(256, (4 * (int64_t)rsel + nprobe - 1) / nprobe);
-    S = std::max<int64_t>(16, std::min<int64_t>(S, h->cap / (2 * (int64_t)nprobe)));  // pass 0 fits the lists
-    const int64_t CH = 256;
-    auto pass_bounds = [&](int p) -> std::pair<int64_t, int64_t> {
-        if (p == 0) return {0, S};
-        int64_t lo = S, hi = S * 8;
-        for (int i = 1; i < p; ++i) {
-            lo = hi;
-            hi = hi * 8;
-        }
-        return {lo, hi};
-    };
-    // Items are counted per pass first and then written in place (one array,
-    // pass-major, each pass in (list, query group, rows) order): the host
-    // builds them while the GPU waits, ~190K items at 512 queries x nprobe 256.
-    std::vector<int64_t> pcount;
-    std::vector<double> pbytes, pflops;
-    for (int l = 0; l < h->nlist; ++l) {
-        const auto& ql = qlist[(size_t)l];
-        if (ql.empty()) continue;
-        const int64_t len = h->list_off[(size_t)l + 1] - h->list_off[(size_t)l];
-        const int64_t ngroups = ((int64_t)ql.size() + 15) / 16;
-        for (int p = 0;; ++p) {
-            auto [lo, hi] = pass_bounds(p);
-            if (lo >= len) break;
-            if ((int)pcount.size() <= p) {
-                pcount.push_back(0);
-                pbytes.push_back(0);
-                pflops.push_back(0);
-            }
-            const int64_t rows = std::min<int64_t>(len, hi) - lo;
-            pcount[(size_t)p] += ngroups * ((rows + CH - 1) / CH);
-            pbytes[(size_t)p] += (double)rows * h->xs_stride * ngroups;
-            pflops[(size_t)p] += 2.0 * rows * (double)ql.size() * h->d;
-        }
-    }
-    const int npass = (int)pcount.size();
-    std::vector<int> pstart(npass + 1, 0);
-    for (int p = 0; p < npass; ++p) pstart[p + 1] = pstart[p] + (int)pcount[(size_t)p];
-    auto& all = h->ivf_items_host;
-    auto& gq = h->ivf_gq_host;
-    all.resize((size_t)pstart[npass]);
-    gq.clear();
-    std::vector<int> pos(pstart.begin(), pstart.end() - 1);
-    for (int l = 0; l < h->nlist; ++l) {
-        const auto& ql = qlist[(size_t)l];
-        if (ql.empty()) continue;
-        const int64_t b0 = h->list_off[(size_t)l], b1 = h->list_off[(size_t)l + 1];
-        for (size_t g0 = 0; g0 < ql.size(); g0 += 16) {
-            const uint32_t nq = (uint32_t)std::min<size_t>(16, ql.size() - g0);
-            const uint32_t qoff = (uint32_t)gq.size();
-            gq.insert(gq.end(), ql.begin() + (int64_t)g0, ql.begin() + (int64_t)(g0 + nq));
-            for (int p = 0; p < npass; ++p) {
-                auto [lo, hi] = pass_bounds(p);
-                if (b0 + lo >= b1) break;
-                const int64_t e = std::min<int64_t>(b1, b0 + hi);
-                for (int64_t r = b0 + lo; r < e; r += CH)
-                    all[(size_t)pos[p]++] = IvfItem{(uint32_t)r, (uint32_t)std::min<int64_t>(e, r + CH), qoff, nq};
-            }
-        }
-    }
-    if (h->items_cap < (int)all.size()) {
-        dfree(h->d_items);
-        h->d_items = nullptr;
-        h->items_cap = std::max((int)all.size(), 1024);
-        HIP_OR(hipMalloc(&h->d_items, (size_t)h->items_cap * sizeof(IvfItem)), false);
-    }
-    if (h->gq_cap < (int64_t)gq.size() || !h->d_gq) {
+    if (h->plan_B < B) {
+        dfree(h->d_every);
         dfree(h->d_gq);
+        h->d_every = nullptr;
         h->d_gq = nullptr;
-        h->gq_cap = std::max<int64_t>((int64_t)gq.size(), 4096);
-        HIP_OR(hipMalloc(&h->d_gq, (size_t)h->gq_cap * 4), false);
+        HIP_OR(hipMalloc(&h->d_every, (size_t)B), false);
+        HIP_OR(hipMalloc(&h->d_gq, (size_t)B * h->nlist * 4), false);
+        h->plan_B = B;
     }
-    if (!h->d_next || h->next_cap < npass) {
-        dfree(h->d_next);
-        h->d_next = nullptr;
-        h->next_cap = std::max(npass, 16);
-        HIP_OR(hipMalloc(&h->d_next, (size_t)h->next_cap * sizeof(uint32_t)), false);
+    if (!h->d_qcount) {
+        HIP_OR(hipMalloc(&h->d_qcount, (size_t)h->nlist * 4), false);
+        HIP_OR(hipMalloc(&h->d_qoff, (size_t)h->nlist * 4), false);
+        HIP_OR(hipMalloc(&h->d_qfill, (size_t)h->nlist * 4), false);
+        HIP_OR(hipMalloc(&h->d_plan, (size_t)kIvfMaxPass * (h->nlist + 1) * 4), false);
+        HIP_OR(hipMalloc(&h->d_nitems, (size_t)kIvfMaxPass * 4), false);
+        HIP_OR(hipMalloc(&h->d_next, (size_t)kIvfMaxPass * 4), false);
     }
-    if (!all.empty())
-        HIP_OR(hipMemcpyAsync(h->d_items, all.data(), all.size() * sizeof(IvfItem), hipMemcpyHostToDevice, st), false);
-    if (!gq.empty()) HIP_OR(hipMemcpyAsync(h->d_gq, gq.data(), gq.size() * 4, hipMemcpyHostToDevice, st), false);
-    HIP_OR(launch_fill_u32(h->cnt, 0u, B, st), false);
-    HIP_OR(launch_fill_u32(h->ovf, 0u, B, st), false);
-    HIP_OR(launch_fill_f32(h->thr, __builtin_inff(), B, st), false);
-    HIP_OR(launch_fill_u32(h->d_next, 0u, h->next_cap, st), false);
-    CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
+    return ensure_ws(h->cent, B, nprobe);
+}
+
+// Algorithmic bytes and flops of each pass's list-scan launch (timing only):
+// a list's rows in the pass once per group of 16 queries probing it, and
+// 2 * rows * d per query.  Reads the step's per-list query counts back.
+bool ivf_pass_work(Handle* h, int npass, int64_t S, std::vector<double>& pbytes, std::vector<double>& pflops) {
+    std::vector<uint32_t> qc((size_t)h->nlist);
+    HIP_OR(hipMemcpyAsync(qc.data(), h->d_qcount, qc.size() * 4, hipMemcpyDeviceToHost, h->st), false);
+    HIP_OR(hipStreamSynchronize(h->st), false);
+    pbytes.assign((size_t)npass, 0.0);
+    pflops.assign((size_t)npass, 0.0);
+    for (int l = 0; l < h->nlist; ++l) {
+        if (!qc[(size_t)l]) continue;
+        const int64_t len = h->list_off[(size_t)l + 1] - h->list_off[(size_t)l];
+        for (int p = 0; p < npass; ++p) {
+            int64_t lo, hi;
+            ivf_pass_bounds(S, p, lo, hi);
+            const int64_t rows = std::min(len, hi) - lo;
+            if (rows <= 0) break;
+            pbytes[(size_t)p] += (double)rows * h->xs_stride * ((qc[(size_t)l] + 15) / 16);
+            pflops[(size_t)p] += 2.0 * rows * (double)qc[(size_t)l] * h->d;
+        }
+    }
+    return true;
+}
+
+bool enqueue_ivf(Handle* h, int B, int k, int rsel, int nprobe, int64_t* ids_out, float* d_out, Profiler& pr,
+                 std::vector<std::pair<int, int>>& scan_ev, std::vector<std::pair<int, int>>& sel_ev,
+                 std::vector<double>& scan_rows, std::vector<std::string>& scan_names, int& ev_f0, int& ev_f1) {
+    hipStream_t st = h->st;
+    const int kk = (int)std::min<int64_t>(k, h->n);
+    pr.ev_prep0 = pr.ev();
+    HIP_OR(launch_prep_queries(SCAN_SQ8, h->dQ32, B, h->d, h->kdim, h->dQs, h->qn, h->qscale, h->slack, h->maxnorm,
+                               1.f, h->sq_scale, h->sq_offset, st, h->cnt, h->ovf, h->thr),
+           false);
+    // probe: the nprobe nearest centroids in exact order, lower id on ties
+    Handle* c = h->cent;
+    HIP_OR(hipMemcpyAsync(c->dQ32, h->dQ32, (size_t)B * h->d * 4, hipMemcpyDeviceToDevice, st), false);
+    {
+        std::vector<std::pair<int, int>> cs, cl;
+        std::vector<double> cr;
+        std::vector<std::string> cn;
+        int cu = 0, c0 = -1, c1 = -1;
+        Profiler cpr{c, cu, &cs, &cl, &cr, &cn, -1, -1};
+        if (!enqueue_flat(c, B, nprobe, nprobe, h->d_probe, h->d_probe_d, cpr, cs, cl, cr, cn, c0, c1)) return false;
+    }
+    const int64_t S = ivf_first_pass(h, rsel, nprobe);
+    const int npass = ivf_passes(h, S);
+    IvfPlanArgs pa{};
+    pa.probe = h->d_probe;
+    pa.B = B;
+    pa.nprobe = nprobe;
+    pa.nlist = h->nlist;
+    pa.kk = kk;
+    pa.list_off = h->d_list_off;
+    pa.qcount = h->d_qcount;
+    pa.qoff = h->d_qoff;
+    pa.qfill = h->d_qfill;
+    pa.gq = h->d_gq;
+    pa.every = h->d_every;
+    pa.plan = h->d_plan;
+    pa.nitems = h->d_nitems;
+    pa.npass = npass;
+    pa.S = S;
+    pa.chunk = kIvfChunk;
+    HIP_OR(launch_ivf_plan(pa, st), false);
+    HIP_OR(launch_fill_u32(h->d_next, 0u, kIvfMaxPass, st), false);
+    if (h->prof && !ivf_pass_work(h, npass, S, pr.bytes_override, pr.flops_override)) return false;
     pr.ev_prep1 = pr.ev();
+    CandBuf cb{h->cnt, h->ckey, h->crid, h->cap};
     IvfScanArgs ia{};
     ia.codes = h->Xs;
     ia.xstride = h->xs_stride;
@@ -1646,6 +1659,11 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
     ia.qn = h->qn;
     ia.qscale = h->qscale;
     ia.thr = h->thr;
+    ia.list_off = h->d_list_off;
+    ia.qcount = h->d_qcount;
+    ia.qoff = h->d_qoff;
+    ia.nlist = h->nlist;
+    ia.chunk = kIvfChunk;
     ia.gq = h->d_gq;
     ia.cand = cb;
     for (int pass = 0; pass < npass; ++pass) {
@@ -1656,25 +1674,24 @@ bool run_step_ivf(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, 
             const hipError_t le = launch_select_threshold(sa, st);
             pr.disarm(e0, e1);
             HIP_OR(le, false);
-            sel_ev.push_back({e0, e1});
+            if (h->prof) sel_ev.push_back({e0, e1});
         }
-        ia.items = h->d_items + pstart[(size_t)pass];
-        ia.nitems = pstart[(size_t)pass + 1] - pstart[(size_t)pass];
+        ia.plan = h->d_plan + (int64_t)pass * (h->nlist + 1);
+        ia.nitems = h->d_nitems + pass;
         ia.next = h->d_next + pass;
+        ivf_pass_bounds(S, pass, ia.lo, ia.hi);
         int e0, e1;
         pr.arm(e0, e1);
         const hipError_t le = launch_ivf_scan(ia, st);
         pr.disarm(e0, e1);
         HIP_OR(le, false);
-        scan_ev.push_back({e0, e1});
-        scan_rows.push_back(0.0);
-        scan_names.push_back("ivf_scan_kernel");
-        pr.bytes_override.push_back(pbytes[(size_t)pass]);
-        pr.flops_override.push_back(pflops[(size_t)pass]);
+        if (h->prof) {
+            scan_ev.push_back({e0, e1});
+            scan_rows.push_back(0.0);
+            scan_names.push_back("ivf_scan_kernel");
+        }
     }
-    if (!finish_step(h, B, k, kk, rsel, ids_out, d_out, cb, false, pr)) return false;
-    h->last_scanned.insert(h->last_scanned.end(), scanned.begin(), scanned.end());
-    return true;
+    return launch_final(h, B, k, rsel, ids_out, d_out, cb, pr, ev_f0, ev_f1);
 }
 
 // Autotuner: for batch sizes 1, 2, 4, ... up to max_batch, time whole query

@@ -31,6 +31,11 @@ __global__ void gather_rows_f32(const float* src, int64_t stride, const uint32_t
     for (int j = threadIdx.x; j < kdim; j += blockDim.x) dst[i * kdim + j] = s[j];
 }
 
+__global__ void fill_u32_k(uint32_t* p, uint32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void fill_u64(unsigned long long* p, unsigned long long v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -92,8 +97,132 @@ __global__ __launch_bounds__(256) void centroid_update(const double* partial, co
     }
 }
 
+// ------------------------------------------------------------ step plan --
+// Exclusive prefix sum over a 1024-thread block (wave scans, then the wave
+// totals); returns this thread's prefix and the block total in *total.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// Queries per list: each query's probes, or every list when its probes hold
+// fewer than kk rows (lsh_index_optimized.cpp:452-457).
+__global__ __launch_bounds__(256) void ivf_plan_count(IvfPlanArgs a) {
+    __shared__ int64_t part[4];
+    const int q = blockIdx.x;
+    const int64_t* pr = a.probe + (int64_t)q * a.nprobe;
+    int64_t rows = 0;
+    for (int j = threadIdx.x; j < a.nprobe; j += blockDim.x) {
+        const int64_t l = pr[j];
+        if (l >= 0) rows += a.list_off[l + 1] - a.list_off[l];
+    }
+#pragma unroll
+    for (int o = 32; o; o >>= 1) rows += __shfl_xor(rows, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = rows;
+    __syncthreads();
+    const bool every = part[0] + part[1] + part[2] + part[3] < (int64_t)a.kk;
+    if (every) {
+        for (int l = threadIdx.x; l < a.nlist; l += blockDim.x) atomicAdd(&a.qcount[l], 1u);
+    } else {
+        for (int j = threadIdx.x; j < a.nprobe; j += blockDim.x)
+            if (pr[j] >= 0) atomicAdd(&a.qcount[pr[j]], 1u);
+    }
+    if (threadIdx.x == 0) a.every[q] = every ? 1 : 0;
+}
+
+// One workgroup: query offsets per list, then per pass the item prefix.
+__global__ __launch_bounds__(1024) void ivf_plan_scan(IvfPlanArgs a) {
+    __shared__ uint32_t wsum[16];
+    uint32_t carry = 0, total = 0;
+    for (int b = 0; b < a.nlist; b += 1024) {
+        const int i = b + (int)threadIdx.x;
+        const uint32_t v = i < a.nlist ? a.qcount[i] : 0u;
+        const uint32_t ex = block_exclusive_scan(v, wsum, &total);
+        if (i < a.nlist) {
+            a.qoff[i] = carry + ex;
+            a.qfill[i] = 0u;
+        }
+        carry += total;
+    }
+    for (int p = 0; p < a.npass; ++p) {
+        int64_t lo, hi;
+        ivf_pass_bounds(a.S, p, lo, hi);
+        uint32_t* plan = a.plan + (int64_t)p * (a.nlist + 1);
+        carry = 0;
+        for (int b = 0; b < a.nlist; b += 1024) {
+            const int i = b + (int)threadIdx.x;
+            uint32_t v = 0;
+            if (i < a.nlist && a.qcount[i] > 0) {
+                const int64_t len = a.list_off[i + 1] - a.list_off[i];
+                const int64_t rows = (len < hi ? len : hi) - lo;
+                if (rows > 0) v = ((a.qcount[i] + 15u) / 16u) * (uint32_t)((rows + a.chunk - 1) / a.chunk);
+            }
+            const uint32_t ex = block_exclusive_scan(v, wsum, &total);
+            if (i < a.nlist) plan[i] = carry + ex;
+            carry += total;
+        }
+        if (threadIdx.x == 0) {
+            plan[a.nlist] = carry;
+            a.nitems[p] = carry;
+        }
+    }
+}
+
+// Query ids into their lists' slots (the order inside a list does not
+// change the answers: a pass's filter is fixed while it runs).
+__global__ __launch_bounds__(256) void ivf_plan_scatter(IvfPlanArgs a) {
+    const int q = blockIdx.x;
+    if (a.every[q]) {
+        for (int l = threadIdx.x; l < a.nlist; l += blockDim.x) a.gq[a.qoff[l] + atomicAdd(&a.qfill[l], 1u)] = (uint32_t)q;
+    } else {
+        const int64_t* pr = a.probe + (int64_t)q * a.nprobe;
+        for (int j = threadIdx.x; j < a.nprobe; j += blockDim.x) {
+            const int64_t l = pr[j];
+            if (l >= 0) a.gq[a.qoff[l] + atomicAdd(&a.qfill[l], 1u)] = (uint32_t)q;
+        }
+    }
+}
+
 // ------------------------------------------------------------ list scan --
 constexpr int kIvfThreads = 256;
+
+// Work item `it` of a pass: its list l (plan[l] <= it < plan[l + 1], binary
+// search), then the query group and row chunk inside the list.
+__device__ inline IvfItem ivf_item(const IvfScanArgs& a, uint32_t it) {
+    int lo = 0, hi = a.nlist;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.plan[mid] <= it) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t local = it - a.plan[lo];
+    const int64_t b0 = a.list_off[lo], len = a.list_off[lo + 1] - b0;
+    const int64_t e = len < a.hi ? len : a.hi;
+    const uint32_t nch = (uint32_t)((e - a.lo + a.chunk - 1) / a.chunk);
+    const uint32_t grp = local / nch, c = local - grp * nch;
+    IvfItem t;
+    t.row_begin = (uint32_t)(b0 + a.lo + (int64_t)c * a.chunk);
+    t.row_end = (uint32_t)(b0 + e < (int64_t)t.row_begin + a.chunk ? b0 + e : (int64_t)t.row_begin + a.chunk);
+    t.qoff = a.qoff[lo] + 16u * grp;
+    const uint32_t left = a.qcount[lo] - 16u * grp;
+    t.nq = left < 16u ? left : 16u;
+    return t;
+}
 
 __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(IvfScanArgs a) {
     const int lane = threadIdx.x & 63;
@@ -105,8 +234,8 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(IvfScanArgs a) {
         uint32_t it = 0;
         if (lane == 0) it = atomicAdd(a.next, 1u);
         it = __shfl(it, 0);
-        if (it >= (uint32_t)a.nitems) break;
-        const IvfItem item = a.items[it];
+        if (it >= *a.nitems) break;
+        const IvfItem item = ivf_item(a, it);
         // A operand: query n of the group (16 queries per tile)
         const int qa = n < (int)item.nq ? (int)a.gq[item.qoff + n] : -1;
         // epilogue slots: queries 4g + i of the group
@@ -274,13 +403,20 @@ hipError_t launch_centroid_update(const double* partial, const int* seg_first, c
     return hipGetLastError();
 }
 
+hipError_t launch_ivf_plan(const IvfPlanArgs& a, hipStream_t st) {
+    if (a.B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_k, dim3((unsigned)((a.nlist + 255) / 256)), dim3(256), 0, st, a.qcount, 0u,
+                       (int64_t)a.nlist);
+    ANNMI_LAUNCH(ivf_plan_count, dim3((unsigned)a.B), dim3(256), 0, st, a);
+    ANNMI_LAUNCH(ivf_plan_scan, dim3(1), dim3(1024), 0, st, a);
+    ANNMI_LAUNCH(ivf_plan_scatter, dim3((unsigned)a.B), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_ivf_scan(const IvfScanArgs& a, hipStream_t st) {
-    if (a.nitems <= 0) return hipSuccess;
+    // persistent: a few workgroups per CU, every wave pulls items until the
+    // pass's queue (its count is on the device) is empty
     const int ncu = device_cu_count();
-    // persistent: a few workgroups per CU, every wave pulls items until the queue is empty
-    int64_t want = (int64_t)ncu * 8;
-    int64_t need = (a.nitems + 3) / 4;
-    unsigned grid = (unsigned)(want < need ? want : need);
-    ANNMI_LAUNCH(ivf_scan_kernel, dim3(grid), dim3(kIvfThreads), 0, st, a);
+    ANNMI_LAUNCH(ivf_scan_kernel, dim3((unsigned)(ncu * 8)), dim3(kIvfThreads), 0, st, a);
     return hipGetLastError();
 }

@@ -48,3 +48,37 @@ def test_replays_with_device_tensors():
             ids, _ = idx.search_device(Qd[3 * i:3 * i + 3], k)
             torch.cuda.synchronize()
             np.testing.assert_array_equal(ids.cpu().numpy(), o_ids[3 * i:3 * i + 3], err_msg=f"rep {rep} batch {i}")
+
+
+@pytest.mark.parametrize("nprobe", [4, 16])
+def test_ivf_steps_are_captured_and_replayed(nprobe):
+    """IVF steps are planned on the device (probe, queries per list, work
+    items), so they capture like flat steps: the second sighting of a shape
+    is captured and later steps replay it.  Every replay gives the eager
+    answer, with eager work of other shapes and host synchronisations in
+    between, including a query whose probed lists hold fewer than k rows
+    (it scans every list inside the captured step)."""
+    n, d, B, k = 40000, 64, 6, 100
+    print(f"[case] ivf graphs nprobe={nprobe}", flush=True)
+    kw = dict(center_seed=8000, normalize=False, zipf_s=1.1)
+    base = data.mixture(n, d, 96, 0.3, 8001, **kw)
+    Q = data.mixture(6 * B, d, 96, 0.3, 8002, **kw)
+    idx = Index(base, mode="ivf", nlist=128, kmeans_iters=5, rerank=400)
+    ref_ids, ref_d = idx.search(Q, k, nprobe=nprobe)
+    assert np.all(ref_ids >= 0)
+    other = data.mixture(9, d, 96, 0.3, 8003, **kw)
+    for i in range(6):
+        ids, dd = idx.search(Q[i * B:(i + 1) * B], k, nprobe=nprobe)  # 1st eager, 2nd captured, then replays
+        np.testing.assert_array_equal(ids, ref_ids[i * B:(i + 1) * B], err_msg=f"step {i}")
+        np.testing.assert_array_equal(dd, ref_d[i * B:(i + 1) * B], err_msg=f"step {i}")
+        idx.search(other[: 1 + i % 4], k, nprobe=nprobe)
+    assert idx.graph_count >= 1
+    # the probed-lists answer of the library equals brute force over the same lists
+    perm, off, cent = idx.export("perm"), idx.export("list_offsets"), idx.export("centroids")
+    full = Index(base, mode="ivf", nlist=128, kmeans_iters=5, rerank=2048)
+    f_ids, _ = full.search(Q[:8], k, nprobe=nprobe)
+    want_ids, _ = O.ivf_search_exact(base, perm, off, cent, Q[:8], k, nprobe)
+    for i in range(8):
+        probes = O.ivf_probe_exact(cent, Q[i:i + 1], nprobe)[0]
+        if sum(int(off[l + 1] - off[l]) for l in probes) <= 2048:
+            np.testing.assert_array_equal(f_ids[i], want_ids[i])
