@@ -224,6 +224,7 @@ struct Handle {
     uint32_t* d_plan = nullptr;     // [kIvfMaxPass x (nlist + 1)]
     uint32_t* d_nitems = nullptr;   // [kIvfMaxPass]
     uint32_t* d_next = nullptr;     // [kIvfMaxPass]
+    uint32_t* d_coarse = nullptr;   // [kIvfMaxPass x kIvfCoarse]
     int plan_B = 0;
     // a clone shares every index array with its source and owns only its
     // stream and workspace
@@ -363,6 +364,7 @@ void free_handle(Handle* h) {
     dfree(h->d_plan);
     dfree(h->d_nitems);
     dfree(h->d_next);
+    dfree(h->d_coarse);
     if (h->cent) free_handle(h->cent);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->st && h->own_stream) (void)hipStreamDestroy(h->st);
@@ -1529,7 +1531,8 @@ bool ivf_build(Handle* h) {
 //                            S 8^p) against the rsel-th smallest key so far
 //   finalize                 fp32 re-score of rsel candidates, original ids
 constexpr int kIvfMaxPass = 16;
-constexpr int kIvfChunk = 256;  // rows of a list per work item
+constexpr int kIvfChunk = 256;         // rows of a list per work item
+constexpr int kIvfCoarse = 1 << 18;    // coarse item table entries a pass (16M items)
 
 // First-pass rows of each probed list: enough for ~4 rsel candidates over
 // the probes, within half the candidate list.
@@ -1577,6 +1580,7 @@ bool ensure_ivf_ws(Handle* h, int nprobe) {
         HIP_OR(hipMalloc(&h->d_plan, (size_t)kIvfMaxPass * (h->nlist + 1) * 4), false);
         HIP_OR(hipMalloc(&h->d_nitems, (size_t)kIvfMaxPass * 4), false);
         HIP_OR(hipMalloc(&h->d_next, (size_t)kIvfMaxPass * 4), false);
+        HIP_OR(hipMalloc(&h->d_coarse, (size_t)kIvfMaxPass * kIvfCoarse * 4), false);
     }
     return ensure_ws(h->cent, B, nprobe);
 }
@@ -1641,6 +1645,8 @@ bool enqueue_ivf(Handle* h, int B, int k, int rsel, int nprobe, int64_t* ids_out
     pa.every = h->d_every;
     pa.plan = h->d_plan;
     pa.nitems = h->d_nitems;
+    pa.coarse = h->d_coarse;
+    pa.coarse_cap = kIvfCoarse;
     pa.npass = npass;
     pa.S = S;
     pa.chunk = kIvfChunk;
@@ -1678,6 +1684,8 @@ bool enqueue_ivf(Handle* h, int B, int k, int rsel, int nprobe, int64_t* ids_out
         }
         ia.plan = h->d_plan + (int64_t)pass * (h->nlist + 1);
         ia.nitems = h->d_nitems + pass;
+        ia.coarse = h->d_coarse + (int64_t)pass * kIvfCoarse;
+        ia.coarse_cap = kIvfCoarse;
         ia.next = h->d_next + pass;
         ivf_pass_bounds(S, pass, ia.lo, ia.hi);
         int e0, e1;

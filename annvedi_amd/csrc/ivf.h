@@ -32,6 +32,8 @@ struct IvfPlanArgs {
     uint8_t* every;           // [B] the query probes every list
     uint32_t* plan;           // [npass x (nlist + 1)]
     uint32_t* nitems;         // [npass]
+    uint32_t* coarse;         // [npass x coarse_cap]: list of item 64 j of the pass
+    int coarse_cap;
     int npass;
     int64_t S;
     int chunk;
@@ -61,6 +63,8 @@ struct IvfScanArgs {
     // (query group, row chunk) order; *nitems in all
     const uint32_t* plan;
     const uint32_t* nitems;
+    const uint32_t* coarse;   // list of item 64 j, j < coarse_cap
+    int coarse_cap;
     const int64_t* list_off;
     const uint32_t* qcount;
     const uint32_t* qoff;

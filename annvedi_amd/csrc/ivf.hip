@@ -183,6 +183,18 @@ __global__ __launch_bounds__(1024) void ivf_plan_scan(IvfPlanArgs a) {
     }
 }
 
+// For every 64th item of each pass, the list holding it: the scan kernel's
+// item search then runs between two neighbouring entries only.
+__global__ __launch_bounds__(256) void ivf_plan_coarse(IvfPlanArgs a) {
+    const int p = blockIdx.y;
+    const uint32_t* plan = a.plan + (int64_t)p * (a.nlist + 1);
+    uint32_t* co = a.coarse + (int64_t)p * a.coarse_cap;
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < a.nlist; l += gridDim.x * blockDim.x) {
+        const uint32_t b = (plan[l] + 63u) >> 6, e = (plan[l + 1] + 63u) >> 6;
+        for (uint32_t j = b; j < e && j < (uint32_t)a.coarse_cap; ++j) co[j] = (uint32_t)l;
+    }
+}
+
 // Query ids into their lists' slots (the order inside a list does not
 // change the answers: a pass's filter is fixed while it runs).
 __global__ __launch_bounds__(256) void ivf_plan_scatter(IvfPlanArgs a) {
@@ -205,6 +217,13 @@ constexpr int kIvfThreads = 256;
 // search), then the query group and row chunk inside the list.
 __device__ inline IvfItem ivf_item(const IvfScanArgs& a, uint32_t it) {
     int lo = 0, hi = a.nlist;
+    const uint32_t j = it >> 6;
+    if (j < (uint32_t)a.coarse_cap) {
+        // plan[coarse[j]] <= 64 j <= it; item 64 (j + 1), when it exists,
+        // lies in list coarse[j + 1], so it < plan[coarse[j + 1] + 1]
+        lo = (int)a.coarse[j];
+        if (j + 1 < (uint32_t)a.coarse_cap && (j + 1) * 64u < *a.nitems) hi = (int)a.coarse[j + 1] + 1;
+    }
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (a.plan[mid] <= it) lo = mid;
@@ -409,6 +428,7 @@ hipError_t launch_ivf_plan(const IvfPlanArgs& a, hipStream_t st) {
                        (int64_t)a.nlist);
     ANNMI_LAUNCH(ivf_plan_count, dim3((unsigned)a.B), dim3(256), 0, st, a);
     ANNMI_LAUNCH(ivf_plan_scan, dim3(1), dim3(1024), 0, st, a);
+    ANNMI_LAUNCH(ivf_plan_coarse, dim3((unsigned)((a.nlist + 255) / 256), (unsigned)a.npass), dim3(256), 0, st, a);
     ANNMI_LAUNCH(ivf_plan_scatter, dim3((unsigned)a.B), dim3(256), 0, st, a);
     return hipGetLastError();
 }
