@@ -123,14 +123,24 @@ int64_t scan_q_bytes(int kind, int kdim) {
 int tile_cb_of(int kind) { return (int)scan_row_bytes(kind, chunk_of(kind)); }
 
 // Host conversions between the row-major and the tiled layout (n rows of
-// `stride` bytes; the tiled buffer holds whole 16-row tiles).
-void untile_rows(const uint8_t* tiled, uint8_t* rows, int64_t n, int64_t stride, int cb) {
+// `stride` bytes; the tiled buffer holds whole 16-row tiles).  The tiled SQ8
+// copy holds each code as c ^ 0x80 (the int8 c - 128 its MFMAs read); flip
+// is 0x80 for it and 0 for the other types.
+int tile_flip_of(int kind) { return kind == SCAN_SQ8 ? 0x80 : 0; }
+void untile_rows(const uint8_t* tiled, uint8_t* rows, int64_t n, int64_t stride, int cb, int flip) {
     for (int64_t r = 0; r < n; ++r)
         for (int64_t b = 0; b < stride; b += cb) memcpy(rows + r * stride + b, tiled + tiled_off(r, b, stride, cb), (size_t)cb);
+    if (flip)
+        for (int64_t i = 0; i < n * stride; ++i) rows[i] ^= (uint8_t)flip;
 }
-void tile_rows(const uint8_t* rows, uint8_t* tiled, int64_t n, int64_t stride, int cb) {
+void tile_rows(const uint8_t* rows, uint8_t* tiled, int64_t n, int64_t stride, int cb, int flip) {
     for (int64_t r = 0; r < n; ++r)
-        for (int64_t b = 0; b < stride; b += cb) memcpy(tiled + tiled_off(r, b, stride, cb), rows + r * stride + b, (size_t)cb);
+        for (int64_t b = 0; b < stride; b += cb) {
+            uint8_t* dst = tiled + tiled_off(r, b, stride, cb);
+            memcpy(dst, rows + r * stride + b, (size_t)cb);
+            if (flip)
+                for (int i = 0; i < cb; ++i) dst[i] ^= (uint8_t)flip;
+        }
 }
 
 struct Handle {
@@ -1214,7 +1224,7 @@ bool load_mode(Handle* h, const uint8_t* blob) {
         }
         const int64_t n_alloc = (n + 15) / 16 * 16;
         std::vector<uint8_t> t((size_t)(n_alloc * h->xs_stride), 0);
-        tile_rows(blob + hd.off[2], t.data(), n, h->xs_stride, tile_cb_of(h->kind));
+        tile_rows(blob + hd.off[2], t.data(), n, h->xs_stride, tile_cb_of(h->kind), tile_flip_of(h->kind));
         HIP_OR(hipMalloc(&h->Xs, t.size()), false);
         HIP_OR(hipMemcpyAsync(h->Xs, t.data(), t.size(), hipMemcpyHostToDevice, st), false);
         HIP_OR(hipStreamSynchronize(st), false);
@@ -2076,7 +2086,7 @@ int64_t annmi_save(void* hv, void* buf, int64_t bytes) {
             std::vector<uint8_t> t((size_t)(n_alloc * h->xs_stride));
             HIP_OR(hipMemcpyAsync(t.data(), h->Xs, t.size(), hipMemcpyDeviceToHost, h->st), -1);
             HIP_OR(hipStreamSynchronize(h->st), -1);
-            untile_rows(t.data(), out + hd.off[2], h->n, h->xs_stride, tile_cb_of(h->kind));
+            untile_rows(t.data(), out + hd.off[2], h->n, h->xs_stride, tile_cb_of(h->kind), tile_flip_of(h->kind));
             continue;
         }
         int64_t m = 0;  // bytes of the array (sections are padded)
@@ -2242,7 +2252,7 @@ int64_t annmi_export(void* hv, int32_t what, void* out, int64_t bytes) {
         std::vector<uint8_t> t((size_t)(n_alloc * h->xs_stride));
         HIP_OR(hipMemcpyAsync(t.data(), h->Xs, t.size(), hipMemcpyDeviceToHost, h->st), -1);
         HIP_OR(hipStreamSynchronize(h->st), -1);
-        untile_rows(t.data(), static_cast<uint8_t*>(out), h->n, h->xs_stride, tile_cb_of(h->kind));
+        untile_rows(t.data(), static_cast<uint8_t*>(out), h->n, h->xs_stride, tile_cb_of(h->kind), tile_flip_of(h->kind));
         return need;
     }
     HIP_OR(hipMemcpyAsync(out, src, (size_t)need, hipMemcpyDeviceToHost, h->st), -1);

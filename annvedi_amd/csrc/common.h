@@ -151,7 +151,8 @@ enum annmi_wide_cfg : int {
     kWideCfg32R2 = 5,      // 32 x 32 MFMA tiles (SQ8, BF16): 128 x 64, 8 waves
     kWideCfg32R4W4 = 6,    // 32 x 32 MFMA tiles: SQ8 128 x 32 on 8 waves, lean epilogue; BF16 128 x 128 on 4 waves
     kWideCfg32R2W4 = 7,    // 32 x 32 MFMA tiles: SQ8 128 x 64 on 8 waves, lean epilogue; BF16 128 x 64 on 4 waves
-    kWideCfgCount = 8,
+    kWideCfgRL = 8,        // SQ8, d padded to 128 or 256: rows staged in LDS by direct loads, 512 queries in registers
+    kWideCfgCount = 9,
 };
 // Whether a wide configuration fits the LDS for this element type and dim.
 bool wide_cfg_fits(int kind, int kdim, int cfg);

@@ -115,7 +115,10 @@ __global__ __launch_bounds__(256) void enc_sq8(const float* src, int64_t sstride
             float v = fmaf((float)c, scale[j], offset[j]);  // reconstructed value
             s = fmaf(v, v, s);
         }
-        dst[dst_elem(row, j, kdim, 1, cb)] = c;
+        // the tiled scan copy stores c ^ 0x80, i.e. c - 128 as the int8 the
+        // MFMA reads (the query side adds 128 * sum(w) back); row-major
+        // copies (IVF lists) keep the plain codes
+        dst[dst_elem(row, j, kdim, 1, cb)] = cb ? (uint8_t)(c ^ 0x80u) : c;
     }
     s = warp16_sum(s);
     if (l == 0) xn[row] = s;

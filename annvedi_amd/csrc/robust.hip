@@ -85,8 +85,9 @@ __global__ __launch_bounds__(kThreads) void rb_gf_keys(RobustArgs a) {
             for (int u = 0; u < 4; ++u) w[u] = p[u];
             const uint8_t* b = reinterpret_cast<const uint8_t*>(w);
             const int m = a.d - c0 < 64 ? a.d - c0 : 64;
+            const uint8_t flip = a.tiled ? 0x80 : 0;  // the tiled copy stores c ^ 0x80
             for (int j = 0; j < m; ++j) {
-                const float v = fmaf((float)b[j], sc[c0 + j], -qsh[c0 + j]);
+                const float v = fmaf((float)(uint8_t)(b[j] ^ flip), sc[c0 + j], -qsh[c0 + j]);
                 acc = fmaf(v, v, acc);
             }
         }

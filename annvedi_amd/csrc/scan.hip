@@ -104,10 +104,9 @@ struct PolSQ8 {
     typedef i32x4 Acc;
     typedef int El;  // accumulator element
     __device__ static inline void mma(i32x4& acc, const i32x4& a, const i32x4& b) {
-        // codes are uint8 in [0,255]; the MFMA reads signed int8, so flip the
-        // top bit: c - 128.  The query side adds 128*sum(w) back.
-        i32x4 bs = b ^ (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
-        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bs, acc, 0, 0, 0);
+        // codes c in [0,255] are stored as c ^ 0x80, which the MFMA reads as
+        // the signed int8 c - 128.  The query side adds 128*sum(w) back.
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
     }
     __device__ static inline void mma0(i32x4& acc, const i32x4& a, const i32x4& b) {
         acc = (i32x4){0, 0, 0, 0};
@@ -1029,10 +1028,9 @@ struct W32SQ8 {
     typedef int El;
     typedef i32x16 Acc;
     __device__ static inline void mma(Acc& acc, const i32x4& a, const i32x4& b) {
-        // codes are uint8; the MFMA reads int8: c - 128 (the query side adds
+        // codes are stored as c ^ 0x80, the int8 c - 128 (the query side adds
         // 128 * sum(w) back)
-        const i32x4 bs = b ^ (i32x4){(int)0x80808080, (int)0x80808080, (int)0x80808080, (int)0x80808080};
-        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bs, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc, 0, 0, 0);
     }
     static const char* name() { return "sq8"; }
 };
@@ -1611,6 +1609,367 @@ hipError_t launch_i8x32_t(const ScanArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// SQ8 wide scan with the rows staged in LDS and the queries held in
+// registers (kWideCfgRL; 32 x 32 x 32 int8 MFMA, tiled rows, prefilter 3).
+//
+// The kernels above keep a row tile per wave in registers and read the
+// query fragments from LDS for every MFMA: each 1 KB query fragment feeds
+// two 16 x 16 MFMAs, so at the MFMA rate the LDS would have to deliver
+// 128 B/clk per CU, and each wave streams its own rows from L2.  Here a
+// workgroup of 8 waves owns 512 queries, 64 a wave, as register-resident A
+// operands (2 fragments of 32 queries x NKS K steps), and the rows of its
+// chunk pass through a ring of three 64-row stages in LDS, written by
+// direct global-to-LDS loads (global_load_lds_dwordx4, no register round
+// trip) two stages ahead and read by all 8 waves: each 1 KB row fragment read
+// feeds two 32 x 32 MFMAs (64 B/clk per CU at the MFMA rate), and L2 serves
+// each row once per 512 queries instead of once per 128.
+//
+// A stage is 4 consecutive 16-row tiles of the tiled layout, i.e. one
+// contiguous run of 2 * NKS blocks of 1 KB (16 rows x one 64-byte chunk).
+// Inside a block, 16-byte unit u of row r is stored at unit u ^ swz(r) of the
+// row, so the 32 rows x 16 bytes a B-operand read takes from a block hit
+// distinct banks.  Synchronisation per stage: each wave waits for its own
+// loads of the stage (vmcnt), then one barrier publishes the stage and
+// frees the buffer that every wave finished in the previous stage, which
+// then receives the stage two ahead.
+//
+// The accumulators hold the plain integer dots.  Prefilter: for each of the
+// lane's 32 (query, row pair) maxima of dot - D_r (D_r = floor((xn_r - xn_lo)
+// * kap), kap <= 1 / (2 qs) over the wave's queries) against the query's
+// integer bound, as in the other wide kernels; only the elements that pass
+// it take the exact test, dist = xn + qv - 2 qs * dot <= thr, the same
+// expression as the other kernels, so every shape appends the same
+// candidates.  Appends go to the wave's LDS queue (query slot = query within
+// the wave's 64) and are flushed per query.
+//
+// Measured (MI355X, SQ8, 10M rows, the largest launches of a step): d = 256,
+// 4096 queries 1652 vs 1638 TOP/s for the built-in kernel, 1024 queries 1568
+// vs 1558; the B-operand reads are free of bank conflicts (SQ_LDS_BANK_CONFLICT
+// 2.5% of the built-in kernel's).  At d = 384 the 96 query-fragment registers
+// and 64 accumulators leave too few of the 256 registers two waves a SIMD
+// allow (the compiler spills, 1752 vs 1814 TOP/s), so this shape takes padded
+// d = 128 and 256 only.  It is an autotuner candidate, not a default.
+__device__ __forceinline__ int rl_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+
+template <int N>
+__device__ __forceinline__ void rl_wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt field");
+    // s_waitcnt: vmcnt N (bits 3:0 and 15:14), expcnt and lgkmcnt not waited
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+template <int NKS>
+__global__ __launch_bounds__(512) void scan_rl_kernel(ScanArgs a, int qtiles, int64_t chunk_rows, int64_t nrc,
+                                                      int qcap) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr int NW = 8, QF = 2, RS = 2, NST = 3, QT = 64 * NW, SR = 64;
+    constexpr int SB = SR * 32 * NKS;  // stage bytes: 4 tiles x NKS / 2 blocks of 1 KB
+    constexpr int NBW = NKS / 4;       // 1 KB blocks a wave loads per stage (2 * NKS over 8 waves)
+    static_assert(NKS % 4 == 0, "whole blocks per wave");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform: scalar address math
+    const int h = lane >> 5;  // K half of the operands; query quad of the outputs
+    const int j = lane & 31;  // operand index (query of A, row of B); output column (row)
+    const int xcd = blockIdx.x & 7;
+    const int64_t s = blockIdx.x >> 3;
+    const int64_t per = (nrc + 7) / 8;
+    if (s / qtiles >= per) return;
+    const int64_t rc = xcd * per + s / qtiles;
+    if (rc >= nrc) return;
+    const int qtile = (int)(s % qtiles);
+    const int64_t r_begin = a.row_begin + rc * chunk_rows;
+    const int64_t r_end = r_begin + chunk_rows < a.row_end ? r_begin + chunk_rows : a.row_end;
+    const int q0 = qtile * QT;
+    const int q0w = q0 + 64 * wave;  // this wave's queries
+
+    // LDS: stages | row norms of the stages [NST][SR] | thr, qn, qsc [QT] |
+    // integer bounds [QT] | queue counters [NW] (+pad) | flush histograms |
+    // exact-test slots (4 KB a wave) | queues
+    float* xn_s = reinterpret_cast<float*>(smem + NST * SB);
+    float* thr_s = xn_s + NST * SR;
+    float* qn_s = thr_s + QT;
+    float* qsc_s = qn_s + QT;
+    int* bg_s = reinterpret_cast<int*>(qsc_s + QT);
+    uint32_t* cnt_s = reinterpret_cast<uint32_t*>(bg_s + QT);
+    uint32_t* hist_s = cnt_s + 16;
+    int* slot = reinterpret_cast<int*>(hist_s + NW * kFlushSlots) + wave * 1024;
+    uint32_t* queues = reinterpret_cast<uint32_t*>(slot - wave * 1024 + NW * 1024);
+    WideQueue wq;
+    wq.row = queues + (int64_t)wave * 2 * qcap;
+    wq.key = reinterpret_cast<float*>(wq.row + qcap);
+    wq.count = cnt_s + wave;
+    wq.hist = hist_s + wave * kFlushSlots;
+    wq.cap = qcap;
+
+    const int64_t nrows = r_end - r_begin;
+    const int nst = (int)((nrows + SR - 1) / SR);
+    const uint8_t* X = static_cast<const uint8_t*>(a.X);
+    const int64_t tile_bytes = 16 * (int64_t)a.xstride;
+    const int64_t t_first = r_begin >> 4;
+    const int64_t t_last = (r_end - 1) >> 4;
+    // direct loads of stage st into its ring buffer: block b = wave + 8 i
+    // (tile b / (NKS/2) of the stage, chunk b % (NKS/2)); lane l writes unit l
+    // of the block, i.e. row l / 4, unit (l % 4) ^ swz(row) of the source;
+    // tiles past the chunk repeat its last tile (their rows are masked)
+    const int dr = lane >> 2;
+    const uint32_t loff = (uint32_t)(dr * 64 + 16 * ((lane & 3) ^ rl_swz(dr)));
+    // The stage's row norms travel the same way (each wave loads 8 of the
+    // 64 with one masked dword load), so the loop has no loads into
+    // registers and every wave issues NBW + 1 loads per stage.
+    auto issue = [&](int st) {
+        const int sb = st % NST;
+        uint8_t* dst = smem + sb * SB;
+#pragma unroll
+        for (int i = 0; i < NBW; ++i) {
+            const int b = wave + 8 * i;
+            int64_t T = t_first + 4 * (int64_t)st + b / (NKS / 2);
+            T = T < t_last ? T : t_last;
+            // the block's address is wave-uniform: kept in scalar registers
+            // (read back through readfirstlane, so it is not folded with the
+            // lane offset into per-lane pointers held across the loop)
+            const uint64_t ub = (uint64_t)(X + T * tile_bytes + (b % (NKS / 2)) * 1024);
+            const uint64_t us = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(ub >> 32)) << 32) |
+                                (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ub);
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(us + loff),
+                                             (__attribute__((address_space(3))) void*)(dst + b * 1024), 16, 0, 0);
+        }
+        if ((lane >> 3) == wave) {
+            // row r0 + lane, clamped to the chunk; the base is scalar
+            const int64_t r0s = r_begin + (int64_t)st * SR;
+            const int64_t lim = r_end - 1 - r0s;  // >= lane for all but the last stage
+            const uint64_t ub = (uint64_t)(a.xn + r0s);
+            const uint64_t us = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(ub >> 32)) << 32) |
+                                (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ub);
+            const int64_t li = (int64_t)lane < lim ? (int64_t)lane : lim;
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(us + 4 * (uint64_t)li),
+                                             (__attribute__((address_space(3))) void*)(xn_s + sb * SR), 4, 0, 0);
+        }
+    };
+    // start the first two stages before the query set-up.  Every iteration
+    // issues one stage (past the last stage: a repeat of the last tile into
+    // the free buffer), so the loop's wait counts are the same every time.
+    issue(0);
+    issue(1);
+
+    // per-query words of the workgroup's 512 queries (one a thread)
+    {
+        const int m = threadIdx.x;
+        const int qi = q0 + m;
+        const bool ok = qi < a.B;
+        const float th = ok ? a.thr[qi] : -__builtin_inff();
+        const float qv = ok ? a.qn[qi] : 0.f;
+        const float qs = ok ? a.qscale[qi] : 0.f;
+        thr_s[m] = th;
+        qn_s[m] = qv;
+        qsc_s[m] = qs;
+        // integer bound of the prefilter: dot >= ((xn_lo + qv) - th - margin)
+        // / (2 qs) is necessary for dist <= th (the wide_stage form)
+        const float M = a.xn_amax + fabsf(qv) + fabsf(th);
+        const float sc = 2.0f * qs;
+        const float num = ((a.xn_lo + qv) - th) - M * 0x1p-14f;
+        const float bg = sc > 0.f ? num / sc : (num <= 0.f ? -__builtin_inff() : __builtin_inff());
+        bg_s[m] = bg != bg ? 0x7fffffff
+                           : (bg <= -2147483648.0f ? (int)0x80000000
+                                                   : (bg >= 2147483647.0f ? 0x7fffffff : (int)ceilf(bg)));
+        if (m < NW) cnt_s[m] = 0u;
+    }
+    // query operands: lane (j, h) of fragment f holds query q0w + 32 f + j,
+    // bytes [32 k + 16 h, +16) of K step k
+    i32x4 qreg[QF][NKS];
+#pragma unroll
+    for (int f = 0; f < QF; ++f) {
+        const int qi = q0w + 32 * f + j;
+        const uint8_t* qp = static_cast<const uint8_t*>(a.Q) + (int64_t)(qi < a.B ? qi : 0) * a.kdim + 16 * h;
+#pragma unroll
+        for (int k = 0; k < NKS; ++k)
+            qreg[f][k] = qi < a.B ? *reinterpret_cast<const i32x4*>(qp + 32 * k) : (i32x4){0, 0, 0, 0};
+    }
+    __syncthreads();
+    // kap over the wave's queries (padding queries have threshold -inf)
+    float kap;
+    {
+        float mq = thr_s[64 * wave + lane] != -__builtin_inff() ? qsc_s[64 * wave + lane] : 0.f;
+#pragma unroll
+        for (int o = 32; o; o >>= 1) mq = fmaxf(mq, __shfl_xor(mq, o));
+        kap = (mq > 0.f && mq < __builtin_inff()) ? (0.5f / mq) * (1.0f - 0x1p-20f) : 0.f;
+    }
+    // bounds of the lane's outputs (read from LDS where used): element e of
+    // fragment f is query 32 f + 8 (e / 4) + 4 h + e % 4 of the wave
+    const int* bgl = bg_s + 64 * wave + 4 * h;
+    // B-operand offsets in a stage: row fragment rf holds rows 32 rf + j,
+    // i.e. tile 2 rf + j / 16, row j % 16; K step k reads chunk k / 2, unit
+    // 2 (k % 2) + h (stored at that unit ^ swz(row))
+    // Two lane offsets cover every read: even and odd K steps (unit h or
+    // 2 + h, i.e. the even one ^ 32 after the swizzle); the row fragment
+    // (NKS * 1024 bytes apart) and the chunk are immediate offsets.
+    const int rr = j & 15;
+    const uint32_t bl0 = (uint32_t)((j >> 4) * (NKS / 2) * 1024 + rr * 64 + 16 * (h ^ rl_swz(rr)));
+    const uint32_t bl1 = bl0 ^ 32u;
+    // every load so far (query operands, the first two stages) complete, so
+    // the loop's own waits count only the loop's loads
+    rl_wait_vm<0>();
+
+    for (int st = 0; st < nst; ++st) {
+        // own loads of stage st done (those of stage st + 1 may be in
+        // flight), then the barrier: every wave's are, and every wave is
+        // past stage st - 1, whose buffer takes stage st + 2
+        rl_wait_vm<NBW + 1>();
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        const int64_t r0 = r_begin + (int64_t)st * SR;
+        __builtin_amdgcn_sched_barrier(0);
+        issue(st + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint8_t* buf0 = smem + (st % NST) * SB + bl0;
+        const uint8_t* buf1 = smem + (st % NST) * SB + bl1;
+        i32x16 acc[QF][RS];
+        // row fragments one K step ahead of their MFMAs; the scheduling
+        // barrier per step keeps the compiler from reading further ahead,
+        // which would take registers the query fragments need
+        i32x4 b[2][RS];
+#pragma unroll
+        for (int rf = 0; rf < RS; ++rf) b[0][rf] = *reinterpret_cast<const i32x4*>(buf0 + rf * NKS * 1024);
+#pragma unroll
+        for (int k = 0; k < NKS; ++k) {
+            if (k + 1 < NKS) {
+#pragma unroll
+                for (int rf = 0; rf < RS; ++rf)
+                    b[(k + 1) & 1][rf] = *reinterpret_cast<const i32x4*>(((k + 1) & 1 ? buf1 : buf0) + rf * NKS * 1024 +
+                                                                         ((k + 1) >> 1) * 1024);
+            }
+#pragma unroll
+            for (int f = 0; f < QF; ++f)
+#pragma unroll
+                for (int rf = 0; rf < RS; ++rf)
+                    acc[f][rf] = __builtin_amdgcn_mfma_i32_32x32x32_i8(qreg[f][k], b[k & 1][rf],
+                                                                       k == 0 ? (i32x16){} : acc[f][rf], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // prefilter (dot - D_r against the query's bound, max over the
+        // lane's two rows)
+        float xn[RS];
+#pragma unroll
+        for (int rf = 0; rf < RS; ++rf) xn[rf] = xn_s[(st % NST) * SR + 32 * rf + j];
+        int dofs[RS];
+#pragma unroll
+        for (int rf = 0; rf < RS; ++rf) dofs[rf] = wide_row_offset(a, xn[rf], kap);
+        uint32_t pend = 0;
+#pragma unroll
+        for (int f = 0; f < QF; ++f) {
+            bool maybe = false;
+#pragma unroll
+            for (int e4 = 0; e4 < 4; ++e4) {
+                const i32x4 bv = *reinterpret_cast<const i32x4*>(bgl + 32 * f + 8 * e4);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int v0 = acc[f][0][4 * e4 + i] - dofs[0];
+                    const int v1 = acc[f][1][4 * e4 + i] - dofs[1];
+                    maybe |= (v0 > v1 ? v0 : v1) >= bv[i];
+                }
+            }
+            pend |= maybe ? 1u << f : 0u;
+        }
+        // exact test of the pairs that pass the prefilter: in a fragment
+        // where some lane may pass, each lane marks its elements that pass,
+        // parks the fragment's 16 dots in its LDS slot and tests the marked
+        // ones one at a time (a loop of as many turns as the most any lane
+        // marked, usually one or two; not unrolled, so the rare appends do
+        // not multiply the code and its live registers)
+        if (__ballot(pend != 0u) != 0) {
+#pragma unroll
+            for (int f = 0; f < QF; ++f) {
+                if (__ballot((pend >> f) & 1u) == 0) continue;
+#pragma unroll
+                for (int rf = 0; rf < RS; ++rf) {
+                    const int64_t row = r0 + 32 * rf + j;
+                    uint32_t mk = 0;
+#pragma unroll
+                    for (int e4 = 0; e4 < 4; ++e4) {
+                        const i32x4 bv = *reinterpret_cast<const i32x4*>(bgl + 32 * f + 8 * e4);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            mk |= (acc[f][rf][4 * e4 + i] - dofs[rf] >= bv[i]) ? 1u << (4 * e4 + i) : 0u;
+                    }
+                    if (row >= r_end) mk = 0;
+                    if (__ballot(mk != 0u) == 0) continue;
+#pragma unroll
+                    for (int e4 = 0; e4 < 4; ++e4)
+                        *reinterpret_cast<i32x4*>(slot + (e4 * 64 + lane) * 4) =
+                            (i32x4){acc[f][rf][4 * e4], acc[f][rf][4 * e4 + 1], acc[f][rf][4 * e4 + 2],
+                                    acc[f][rf][4 * e4 + 3]};
+                    const float xr = xn[rf];
+#pragma unroll 1
+                    while (__ballot(mk != 0u) != 0) {
+                        if (mk != 0u) {
+                            const int e = __builtin_ctz(mk);
+                            mk &= mk - 1u;
+                            const int m = 32 * f + 8 * (e >> 2) + 4 * h + (e & 3);
+                            const int mw = 64 * wave + m;
+                            const float th = thr_s[mw], qv = qn_s[mw], sc = 2.0f * qsc_s[mw];
+                            const float dot = (float)slot[((e >> 2) * 64 + lane) * 4 + (e & 3)];
+                            const float dist = xr + qv - sc * dot;
+                            if (dist <= th)
+                                wide_append(a, wq, q0w + m, m, (uint32_t)(row - r_begin), (uint32_t)row, dist);
+                        }
+                    }
+                }
+            }
+        }
+        if (__builtin_amdgcn_readfirstlane(*wq.count) + 256u > (uint32_t)qcap) wide_flush(a, wq, q0w, r_begin, lane);
+    }
+    // the repeat loads of the last two iterations land before the
+    // workgroup's LDS is released
+    rl_wait_vm<0>();
+    wide_flush(a, wq, q0w, r_begin, lane);
+}
+
+template <int NKS>
+hipError_t launch_rl_t(const ScanArgs& a, hipStream_t st) {
+    static const char* const name = [] {
+        static char buf[80];
+        snprintf(buf, sizeof(buf), "scan_rl_kernel<sq8,NKS=%d>", NKS);
+        return buf;
+    }();
+    t_variant = name;
+    constexpr int NW = 8, NST = 3, QT = 512, SR = 64;
+    if (!a.tiled || a.prefilter != 3 || a.kdim != 32 * NKS || a.xstride != a.kdim || !a.qscale)
+        return hipErrorInvalidValue;
+    const size_t fixed = (size_t)NST * SR * 32 * NKS + (size_t)NST * SR * 4 + (size_t)QT * 16 + 64 +
+                         (size_t)NW * kFlushSlots * 4 + (size_t)NW * 4096;
+    if (fixed + (size_t)NW * 256 * 8 > 160 * 1024) return hipErrorInvalidValue;
+    int qcap = (int)((160 * 1024 - fixed) / (NW * 8)) / 64 * 64;
+    if (qcap > 2048) qcap = 2048;
+    if (const char* e = getenv("ANNMI_WIDE_QCAP")) {  // lower cap: tuning and tests of the spill path
+        const int v = atoi(e);
+        if (v >= 0 && v < qcap) qcap = v / 64 * 64;
+    }
+    static const bool attr = [] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_rl_kernel<NKS>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        return true;
+    }();
+    (void)attr;
+    const int ncu = device_cu_count();
+    const int qtiles = (a.B + QT - 1) / QT;
+    const int64_t rows = a.row_end - a.row_begin;
+    static const int64_t chunk_env = [] {
+        const char* e = getenv("ANNMI_WIDE_CHUNK_MAX");
+        const long v = e ? atol(e) : 0;
+        return v >= 2048 ? (int64_t)v : (int64_t)32768;
+    }();
+    const int64_t chunk_max = a.chunk_max >= 2048 ? a.chunk_max : chunk_env;
+    // one workgroup a CU at a time; about 8 a CU over the launch
+    int64_t chunk = rows * qtiles / ((int64_t)ncu * 8);
+    chunk = chunk < 1024 ? 1024 : (chunk > chunk_max ? chunk_max : chunk);
+    chunk = (chunk + SR - 1) / SR * SR;
+    const int64_t nrc = (rows + chunk - 1) / chunk;
+    const int64_t per = (nrc + 7) / 8;
+    ANNMI_LAUNCH((scan_rl_kernel<NKS>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
+                 fixed + (size_t)NW * qcap * 8, st, a, qtiles, chunk, nrc, qcap);
+    return hipGetLastError();
+}
+
 // LDS of a wide configuration's query tile: 16*QS queries of qrow bytes plus
 // five per-query floats (the queues take what is left of the 160 KB).
 template <class P>
@@ -1648,6 +2007,13 @@ hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
     typedef typename W32Of<P>::T W;
     if constexpr (!std::is_void<W>::value) {
         if constexpr (std::is_same<P, PolSQ8>::value) {
+            if (a.wide_cfg == kWideCfgRL) {
+                switch (a.kdim) {
+                    case 128: return launch_rl_t<4>(a, st);
+                    case 256: return launch_rl_t<8>(a, st);
+                    default: return hipErrorInvalidValue;
+                }
+            }
             if ((a.wide_cfg == kWideCfg32R4W4 || a.wide_cfg == kWideCfg32R2W4) && a.tiled && a.prefilter == 3) {
                 const bool d4 = nks_of<W>(a) % 4 == 0;
                 if (a.wide_cfg == kWideCfg32R4W4)
@@ -1753,6 +2119,7 @@ hipError_t launch_scan(const ScanArgs& a, hipStream_t st, const char** variant) 
 bool wide_cfg_fits(int kind, int kdim, int cfg) {
     const int qs = cfg == kWideCfgQ4R4W8 ? 4 : 8;
     if (cfg <= kWideCfgDefault || cfg >= kWideCfgCount) return false;
+    if (cfg == kWideCfgRL) return kind == SCAN_SQ8 && (kdim == 128 || kdim == 256);
     if (cfg == kWideCfg32R1 || cfg == kWideCfg32R2 || cfg == kWideCfg32R4W4 ||
         cfg == kWideCfg32R2W4) {  // 128 queries, whole 64-byte row chunks
         if (kind != SCAN_SQ8 && kind != SCAN_BF16) return false;

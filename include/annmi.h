@@ -209,7 +209,8 @@ int32_t annmi_autotune(void* h, int32_t max_batch, int32_t reps, char* report, i
  * waves; 3: 64 x 64 on 8 waves; SQ8 and BF16 only, 32 x 32 MFMA tiles:
  * 4: 128 x 32 on 8 waves, 5: 128 x 64 on 8 waves, 6 and 7: SQ8 128 x 32 and
  * 128 x 64 on 8 waves with a compact epilogue, BF16 128 x 128 and 128 x 64
- * on 4 waves) and at most chunk_rows
+ * on 4 waves; 8: SQ8 with d padded to 128 or 256 only, 512 queries in
+ * registers over rows staged in LDS) and at most chunk_rows
  * rows per row chunk
  * (0: built-in, else >= 2048).  Returns 0, or -1 for a configuration that does
  * not fit the LDS at this dim.  No reference counterpart (tuning, tests). */

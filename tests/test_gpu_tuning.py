@@ -28,7 +28,7 @@ def test_every_scan_shape_gives_the_same_answers(inputs, mode):
     ref_small = [idx.search(Q[:b], 100) for b in (1, 3)]  # tiny batches: streaming or wide by type
     if mode == "exact":
         np.testing.assert_array_equal(ref_ids, O.exact_topk_f64(base, Q, 100)[0])
-    shapes = [(1 << 30, 0, 0), (1, 0, 0), (1, 0, 4096)] + [(1, c, ch) for c in (1, 2, 3, 4, 5, 6, 7) for ch in (0, 8192)]
+    shapes = [(1 << 30, 0, 0), (1, 0, 0), (1, 0, 4096)] + [(1, c, ch) for c in (1, 2, 3, 4, 5, 6, 7, 8) for ch in (0, 8192)]
     tried = 0
     for mb, cfg, ch in shapes:
         try:
@@ -61,4 +61,4 @@ def test_autotune_table_keeps_answers(inputs):
         np.testing.assert_array_equal(ids, ref_ids[:B])
         np.testing.assert_array_equal(dd, ref_d[:B])
     with pytest.raises(RuntimeError):
-        idx.set_scan_shape(1, 8, 0)  # past the last configuration
+        idx.set_scan_shape(1, 9, 0)  # past the last configuration
