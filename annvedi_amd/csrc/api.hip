@@ -184,6 +184,7 @@ struct Handle {
     uint32_t* crid = nullptr;
     uint32_t* ovf = nullptr;
     uint32_t* nrr = nullptr;
+    uint32_t* hpin = nullptr;  // pinned host words: a step's flags, re-score counts, probe flags (3 x ws_B)
     int64_t* d_ids = nullptr;
     float* d_dist = nullptr;
     uint32_t* scratch_keys = nullptr;
@@ -256,11 +257,14 @@ struct Handle {
     // is dropped whenever the workspace is reallocated.
     struct StepGraph {
         int B, k, rsel, nprobe;
-        const void* ids;
+        const void* ids;      // output buffers the graph writes (its finalize node's arguments)
         const void* dist;
         hipGraphExec_t exec;  // null: seen once, not captured yet
         bool failed;          // capture or instantiation failed: run eagerly
         uint64_t last_use;
+        hipGraph_t graph;     // the captured graph (kept: its finalize node is retargeted)
+        hipGraphNode_t fin;   // the finalize kernel node (null: none found, not retargetable)
+        FinalArgs fa;         // its arguments as captured
     };
     std::vector<StepGraph> graphs;
     uint64_t graph_clock = 0;
@@ -300,9 +304,69 @@ void apply_shape(const Handle::ScanShape& sh, ScanArgs& a) {
 }
 
 void drop_graphs(Handle* h) {
-    for (auto& g : h->graphs)
+    for (auto& g : h->graphs) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     h->graphs.clear();
+}
+
+// The finalize kernel node of a captured step that writes the caller's
+// output buffer `ids` (an IVF step also holds the probe's finalize, which
+// writes the handle's own probe buffers), and its arguments.
+bool find_final_node(hipGraph_t g, const int64_t* ids, hipGraphNode_t* node, FinalArgs* fa) {
+    size_t nn = 0;
+    if (hipGraphGetNodes(g, nullptr, &nn) != hipSuccess || nn == 0) return false;
+    std::vector<hipGraphNode_t> nodes(nn);
+    if (hipGraphGetNodes(g, nodes.data(), &nn) != hipSuccess) return false;
+    for (size_t i = 0; i < nn; ++i) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nodes[i], &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams kp{};
+        if (hipGraphKernelNodeGetParams(nodes[i], &kp) != hipSuccess) continue;
+        if (kp.func != finalize_kernel_fn() || !kp.kernelParams || !kp.kernelParams[0]) continue;
+        FinalArgs f;
+        std::memcpy(&f, kp.kernelParams[0], sizeof(FinalArgs));
+        if (f.out_ids != ids) continue;
+        *node = nodes[i];
+        *fa = f;
+        return true;
+    }
+    return false;
+}
+
+// Points a captured step at other output buffers: the finalize node's
+// arguments in the executable graph are replaced (no other node writes
+// them), so steps of one shape share one graph whatever buffers the caller
+// passes.
+bool graph_debug() {  // ANNMI_GRAPH_DEBUG=1: report why a step shape runs eagerly (stderr)
+    static const bool on = getenv("ANNMI_GRAPH_DEBUG") != nullptr;
+    return on;
+}
+
+bool retarget_graph(Handle::StepGraph& g, int64_t* ids, float* dist) {
+    if (!g.fin) {
+        if (graph_debug()) fprintf(stderr, "annmi graphs: B=%d no finalize node to retarget\n", g.B);
+        return false;
+    }
+    hipKernelNodeParams kp{};
+    hipError_t e = hipGraphKernelNodeGetParams(g.fin, &kp);
+    if (e == hipSuccess) {
+        FinalArgs fa = g.fa;
+        fa.out_ids = ids;
+        fa.out_d = dist;
+        void* args[1] = {&fa};
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        e = hipGraphExecKernelNodeSetParams(g.exec, g.fin, &kp);
+    }
+    if (e != hipSuccess) {
+        if (graph_debug()) fprintf(stderr, "annmi graphs: B=%d retarget failed: %s\n", g.B, hipGetErrorString(e));
+        return false;
+    }
+    g.ids = ids;
+    g.dist = dist;
+    return true;
 }
 
 void free_ws(Handle* h) {
@@ -320,6 +384,8 @@ void free_ws(Handle* h) {
     dfree(h->nrr);
     dfree(h->d_ids);
     dfree(h->d_dist);
+    if (h->hpin) (void)hipHostFree(h->hpin);
+    h->hpin = nullptr;
     h->dQ32 = nullptr;
     h->dQs = nullptr;
     h->qn = h->qscale = h->slack = h->thr = nullptr;
@@ -398,6 +464,9 @@ bool ensure_ws(Handle* h, int B, int k) {
     HIP_OR(hipMalloc(&h->nrr, Bq * sizeof(uint32_t)), false);
     HIP_OR(hipMalloc(&h->d_ids, Bq * 2048 * sizeof(int64_t)), false);
     HIP_OR(hipMalloc(&h->d_dist, Bq * 2048 * sizeof(float)), false);
+    // the per-step read-back lands in pinned memory: an asynchronous copy,
+    // not a staged one that waits on the stream by itself
+    HIP_OR(hipHostMalloc(reinterpret_cast<void**>(&h->hpin), 3 * Bq * sizeof(uint32_t), hipHostMallocDefault), false);
     h->ws_B = B;
     (void)k;
     return true;
@@ -594,15 +663,15 @@ bool run_robust(Handle* h, const std::vector<int32_t>& qs, int k, int64_t* ids_o
 bool complete_step(Handle* h, int B, int k, int64_t* ids_out, float* d_out, bool exact, Profiler& pr, int ev_f0,
                    int ev_f1, const uint32_t* probe_flags) {
     hipStream_t st = h->st;
-    std::vector<uint32_t> flags(B), nrr(B), pflags;
-    HIP_OR(hipMemcpyAsync(flags.data(), h->ovf, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
-    HIP_OR(hipMemcpyAsync(nrr.data(), h->nrr, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
-    if (probe_flags) {  // the IVF probe's own flags: a query whose probe could not be bounded
-        pflags.resize(B);
-        HIP_OR(hipMemcpyAsync(pflags.data(), probe_flags, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
-    }
+    uint32_t* hf = h->hpin;
+    HIP_OR(hipMemcpyAsync(hf, h->ovf, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
+    HIP_OR(hipMemcpyAsync(hf + B, h->nrr, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
+    if (probe_flags)  // the IVF probe's own flags: a query whose probe could not be bounded
+        HIP_OR(hipMemcpyAsync(hf + 2 * B, probe_flags, B * sizeof(uint32_t), hipMemcpyDeviceToHost, st), false);
     HIP_OR(hipStreamSynchronize(st), false);
-    for (size_t i = 0; i < pflags.size(); ++i) flags[i] |= pflags[i] & 7u;
+    std::vector<uint32_t> flags(hf, hf + B), nrr(hf + B, hf + 2 * B);
+    if (probe_flags)
+        for (int i = 0; i < B; ++i) flags[i] |= hf[2 * B + i] & 7u;
     if (h->prof && ev_f1 >= 0) {
         auto& scan_ev = *pr.scan_ev;
         auto& sel_ev = *pr.sel_ev;
@@ -794,18 +863,21 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
     const uint32_t* probe_flags = h->ivf ? h->cent->ovf : nullptr;
     Handle::StepGraph* sg = nullptr;
     if (graphs_enabled() && !h->prof && !h->tuning && !h->shared.load()) {
+        // one graph per step shape: a graph captured for other output
+        // buffers is retargeted before it is replayed
         for (auto& g : h->graphs)
-            if (g.B == B && g.k == k && g.rsel == rsel && g.nprobe == nprobe && g.ids == ids_out && g.dist == d_out)
-                sg = &g;
+            if (g.B == B && g.k == k && g.rsel == rsel && g.nprobe == nprobe) sg = &g;
         if (!sg) {
             if (h->graphs.size() >= kMaxGraphs) {  // evict the least recently used
                 size_t lru = 0;
                 for (size_t i = 1; i < h->graphs.size(); ++i)
                     if (h->graphs[i].last_use < h->graphs[lru].last_use) lru = i;
                 if (h->graphs[lru].exec) (void)hipGraphExecDestroy(h->graphs[lru].exec);
+                if (h->graphs[lru].graph) (void)hipGraphDestroy(h->graphs[lru].graph);
                 h->graphs.erase(h->graphs.begin() + (long)lru);
             }
-            h->graphs.push_back(Handle::StepGraph{B, k, rsel, nprobe, ids_out, d_out, nullptr, false, ++h->graph_clock});
+            h->graphs.push_back(Handle::StepGraph{B, k, rsel, nprobe, ids_out, d_out, nullptr, false,
+                                                  ++h->graph_clock, nullptr, nullptr, FinalArgs{}});
         } else {
             sg->last_use = ++h->graph_clock;
             if (!sg->exec && !sg->failed) {
@@ -819,17 +891,39 @@ bool run_step(Handle* h, int B, int k, int R, int nprobe, int64_t* ids_out, floa
                     return false;
                 }
                 hipGraphExec_t exec = nullptr;
-                if (ec == hipSuccess && graph &&
-                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+                const hipError_t ei = (ec == hipSuccess && graph)
+                                          ? hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)
+                                          : hipErrorUnknown;
+                if (graph_debug() && ei != hipSuccess)
+                    fprintf(stderr, "annmi graphs: B=%d capture %s, instantiate %s\n", B, hipGetErrorString(ec),
+                            hipGetErrorString(ei));
+                if (ei == hipSuccess) {
                     sg->exec = exec;
-                else
+                    sg->graph = graph;
+                    sg->ids = ids_out;
+                    sg->dist = d_out;
+                    if (!find_final_node(graph, ids_out, &sg->fin, &sg->fa)) {
+                        sg->fin = nullptr;
+                        if (graph_debug()) fprintf(stderr, "annmi graphs: B=%d finalize node not found\n", B);
+                    }
+                } else {
                     sg->failed = true;
-                if (graph) (void)hipGraphDestroy(graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                }
                 (void)hipGetLastError();
                 scan_ev.clear();
                 sel_ev.clear();
                 scan_rows.clear();
                 scan_names.clear();
+            }
+            if (sg->exec && (sg->ids != ids_out || sg->dist != d_out) && !retarget_graph(*sg, ids_out, d_out)) {
+                // not retargetable: this shape runs eagerly from now on
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(sg->exec);
+                if (sg->graph) (void)hipGraphDestroy(sg->graph);
+                sg->exec = nullptr;
+                sg->graph = nullptr;
+                sg->failed = true;
             }
             if (sg->exec) {
                 HIP_OR(hipGraphLaunch(sg->exec, st), false);

@@ -188,3 +188,5 @@ struct FinalArgs {
     double eps_rel;           // FP32 near-tie window (relative) for the FP64 boundary step
 };
 hipError_t launch_finalize(const FinalArgs& a, hipStream_t st);
+// The finalize kernel's function (to find its node in a captured step).
+const void* finalize_kernel_fn();

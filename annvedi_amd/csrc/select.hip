@@ -185,17 +185,23 @@ __global__ __launch_bounds__(kThreads) void finalize(FinalArgs a) {
         ncol = kRMax;
     }
     // 3. fp32 re-score, one row per 16-lane group
-    // (two rows per group at a time, so twice the loads are in flight)
+    // (four rows per group at a time, so four times the loads are in flight:
+    // one query's ~130 rows take two rounds of row loads instead of five)
     const int grp = threadIdx.x >> 4, l16 = threadIdx.x & 15;
     constexpr int G = kThreads / 16;
-    for (uint32_t c = grp; c < ncol; c += 2 * G) {
-        const uint32_t c1 = c + G < ncol ? c + G : c;
-        const float* x[2] = {a.X32 + (int64_t)S.id[c] * a.x32_stride, a.X32 + (int64_t)S.id[c1] * a.x32_stride};
-        float v[2];
-        l2_ref_order<2>(x, qs, a.d, l16, v);
+    for (uint32_t c = grp; c < ncol; c += 4 * G) {
+        uint32_t cr[4];
+        const float* x[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cr[r] = c + r * G < ncol ? c + r * G : c;
+            x[r] = a.X32 + (int64_t)S.id[cr[r]] * a.x32_stride;
+        }
+        float v[4];
+        l2_ref_order<4>(x, qs, a.d, l16, v);
         if (l16 == 0) {
-            S.d32[c] = v[0];
-            S.d32[c1] = v[1];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) S.d32[cr[r]] = v[r];
         }
     }
     __syncthreads();
@@ -276,6 +282,8 @@ hipError_t launch_select_threshold(const SelectArgs& a, hipStream_t st) {
 }
 
 size_t finalize_smem_bytes(int d) { return sizeof(FinalSmem) + (size_t)d * sizeof(float); }
+
+const void* finalize_kernel_fn() { return reinterpret_cast<const void*>(&finalize); }
 
 hipError_t launch_finalize(const FinalArgs& a, hipStream_t st) {
     if (a.B <= 0) return hipSuccess;

@@ -150,15 +150,24 @@ class Index:
             raise RuntimeError(_lib.last_error())
         return ids[:, :got], vals[:, :got]
 
-    def search_device(self, Q, k: int, rerank: int = 0, nprobe: int = 0):
-        """Same for a torch device tensor; returns device tensors."""
+    def search_device(self, Q, k: int, rerank: int = 0, nprobe: int = 0, out=None):
+        """Same for a torch device tensor; returns device tensors.  out: an
+        optional (ids int64 [B, k], dists float32 [B, k]) pair of contiguous
+        device tensors to write (a serving loop that reuses its buffers)."""
         import torch
 
         self._check_q(Q)
         Q = Q.contiguous().to(torch.float32)
         B = Q.shape[0]
-        ids = torch.empty((B, k), dtype=torch.int64, device=Q.device)
-        dd = torch.empty((B, k), dtype=torch.float32, device=Q.device)
+        if out is None:
+            ids = torch.empty((B, k), dtype=torch.int64, device=Q.device)
+            dd = torch.empty((B, k), dtype=torch.float32, device=Q.device)
+        else:
+            ids, dd = out
+            if (tuple(ids.shape) != (B, k) or tuple(dd.shape) != (B, k) or ids.dtype != torch.int64
+                    or dd.dtype != torch.float32 or not ids.is_contiguous() or not dd.is_contiguous()
+                    or ids.device != Q.device or dd.device != Q.device):
+                raise RuntimeError(f"out must be contiguous ({B}, {k}) int64 and float32 tensors on {Q.device}")
         torch.cuda.current_stream().synchronize()
         got = self._L.annmi_batch_query_ex(self._h, ctypes.c_void_p(Q.data_ptr()), B, int(k), int(rerank),
                                            int(nprobe), ctypes.c_void_p(ids.data_ptr()),

@@ -50,6 +50,37 @@ def test_replays_with_device_tensors():
             np.testing.assert_array_equal(ids.cpu().numpy(), o_ids[3 * i:3 * i + 3], err_msg=f"rep {rep} batch {i}")
 
 
+@pytest.mark.parametrize("mode", ["exact", "sq8", "ivf"])
+def test_replays_retargeted_to_new_output_buffers(mode):
+    """One graph per step shape: a replay into output buffers other than the
+    ones it was captured with rewrites its finalize node's arguments.  Every
+    step writes fresh device tensors (all kept alive, so no two share an
+    address) and each equals the eager answer; the buffers of earlier steps
+    keep their answers."""
+    import torch
+
+    n, d, k, B = 50000, 64, 100, 3
+    base = data.mixture(n, d, 48, 0.3, 81, center_seed=82)
+    Q = data.mixture(8 * B, d, 48, 0.3, 83, center_seed=82)
+    print(f"[case] graph retarget mode={mode} n={n} d={d} B={B}", flush=True)
+    kw = dict(nlist=64, nprobe=8) if mode == "ivf" else {}
+    idx = Index(base, mode=mode, rerank=300, **kw)
+    ref_ids, ref_d = idx.search(Q, k)
+    Qd = torch.from_numpy(Q).cuda()
+    outs = []
+    for i in range(8):
+        ids = torch.full((B, k), -7, dtype=torch.int64, device="cuda")
+        dd = torch.full((B, k), -7.0, dtype=torch.float32, device="cuda")
+        got = idx.search_device(Qd[i * B:(i + 1) * B], k, out=(ids, dd))
+        assert got[0].data_ptr() == ids.data_ptr()
+        outs.append((ids, dd))
+        idx.search(Q[:1 + i % 2], k)  # eager work of other shapes in between
+    assert idx.graph_count >= 1
+    for i, (ids, dd) in enumerate(outs):
+        np.testing.assert_array_equal(ids.cpu().numpy(), ref_ids[i * B:(i + 1) * B], err_msg=f"step {i}")
+        np.testing.assert_array_equal(dd.cpu().numpy(), ref_d[i * B:(i + 1) * B], err_msg=f"step {i}")
+
+
 @pytest.mark.parametrize("nprobe", [4, 16])
 def test_ivf_steps_are_captured_and_replayed(nprobe):
     """IVF steps are planned on the device (probe, queries per list, work
