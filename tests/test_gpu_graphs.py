@@ -65,6 +65,7 @@ def test_replays_retargeted_to_new_output_buffers(mode):
     print(f"[case] graph retarget mode={mode} n={n} d={d} B={B}", flush=True)
     kw = dict(nlist=64, nprobe=8) if mode == "ivf" else {}
     idx = Index(base, mode=mode, rerank=300, **kw)
+    other = Index(base, mode="exact")
     ref_ids, ref_d = idx.search(Q, k)
     Qd = torch.from_numpy(Q).cuda()
     outs = []
@@ -74,8 +75,12 @@ def test_replays_retargeted_to_new_output_buffers(mode):
         got = idx.search_device(Qd[i * B:(i + 1) * B], k, out=(ids, dd))
         assert got[0].data_ptr() == ids.data_ptr()
         outs.append((ids, dd))
-        idx.search(Q[:1 + i % 2], k)  # eager work of other shapes in between
-    assert idx.graph_count >= 1
+        # eager work in between: another index's search and torch work
+        other.search(Q[:1 + i % 2], k)
+        float((Qd * 2.0).sum())
+    # the one shape seen here was captured once and replayed, retargeted, six
+    # times (a failed retarget would have dropped the graph)
+    assert idx.graph_count == 1
     for i, (ids, dd) in enumerate(outs):
         np.testing.assert_array_equal(ids.cpu().numpy(), ref_ids[i * B:(i + 1) * B], err_msg=f"step {i}")
         np.testing.assert_array_equal(dd.cpu().numpy(), ref_d[i * B:(i + 1) * B], err_msg=f"step {i}")
