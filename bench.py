@@ -192,7 +192,7 @@ def workload_shape(workload: str, n: int = 0, d: int = 0):
 
 def bench_ivf(args):
     """Served IVF + SQ8: 10M x 768 Zipf(1.1) mixture, k-means nlist = 4096,
-    nprobe in {16, 64, 256}, Poisson arrivals, request sizes lognormal
+    nprobe in {16, 64, 256, 512}, Poisson arrivals, request sizes lognormal
     (median 8, sigma 1, clipped to [1, 512]), continuous batching on 4
     handles sharing the index.  Before serving, the list-scan kernel is timed
     on full 512-query steps per nprobe (its roofline).  Prints one JSON
@@ -225,7 +225,7 @@ def bench_ivf(args):
     # a list (the kernel's unit of work)
     steps = []
     Qstep = Qd[:512]
-    for nprobe in (16, 64, 256):
+    for nprobe in (16, 64, 256, 512):
         idx.search_device(Qstep, k, nprobe=nprobe)  # warm this shape
         torch.cuda.synchronize()
         idx.profile(1)
@@ -245,11 +245,14 @@ def bench_ivf(args):
                       "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
         log(f"[ivf] list scan {steps[-1]}")
     runs = []
-    for nprobe in (16, 64, 256):
+    for nprobe in (16, 64, 256, 512):
         ids, _ = idx.search_device(Qcal, k, nprobe=nprobe)
         rc = float(recall_vs_exact(base, Qcal, ids, gt, k).mean())
-        # 256 probed lists (~6% of the rows a query) serve at lower rates
-        for rate in (args.rates if nprobe < 256 else [x for x in args.rates if x <= 10000]):
+        # 256 probed lists (~6% of the rows a query) serve at lower rates,
+        # 512 (the high-recall point) at 1k and 2k offered
+        rates = args.rates if nprobe < 256 else ([x for x in args.rates if x <= 10000] if nprobe < 512
+                                                 else [1000, 2000])
+        for rate in rates:
             with Server(idx, workers=4, max_batch=512, nprobe=nprobe) as srv:
                 run_open_loop(srv, Qh, sizes[:50], qps=rate, k=k)  # warm the handles
                 r = run_open_loop(srv, Qh, sizes, qps=rate, k=k)

@@ -216,14 +216,17 @@ int32_t annmi_autotune(void* h, int32_t max_batch, int32_t reps, char* report, i
  * not fit the LDS at this dim.  No reference counterpart (tuning, tests). */
 int32_t annmi_set_scan_shape(void* h, int32_t wide_min_batch, int32_t wide_cfg, int64_t chunk_rows);
 
-/* Number of step shapes (batch size, k, re-score count, IVF probe count,
- * output buffers) whose query step this handle replays as a captured HIP
- * graph.  A step -- flat or IVF: the IVF probe, work plan and list scan are
- * all enqueued on the device with no host round trip -- runs eagerly the
- * first time its shape is seen, is captured the second time and replayed
- * from then on (ANNMI_GRAPHS=0 disables this; steps are never captured while
- * annmi_profile timing is on, nor on cloned handles).  No reference
- * counterpart (diagnostics).  -1 on a NULL handle. */
+/* Number of step shapes (batch size, k, re-score count, IVF probe count)
+ * whose query step this handle replays as a captured HIP graph.  A step --
+ * flat or IVF: the IVF probe, work plan and list scan are all enqueued on
+ * the device with no host round trip -- runs eagerly the first time its
+ * shape is seen, is captured the second time and replayed from then on,
+ * into whatever output buffers the call passes (the replay rewrites the
+ * captured finalize kernel's output arguments).  ANNMI_GRAPHS=0 disables
+ * this; steps are never captured while annmi_profile timing is on, nor on
+ * cloned handles; growing the workspace (a larger batch than any before)
+ * drops the captured steps.  No reference counterpart (diagnostics).  -1 on
+ * a NULL handle. */
 int32_t annmi_graph_count(void* h);
 
 /* ------------------------------------------------------------------
