@@ -41,6 +41,12 @@ Parity pins (each a check a plausible mistake would fail):
 * ``f32_direct_l2`` (gpu_filter.cu:161-170): integer-valued rows (every sum
   exact), a sum that only the sequential order absorbs, a product that FMA
   contraction would not round.
+* ``HnswCpu`` (submission/src/hnsw.cpp, oracle/hnsw_cpu.cpp; the CPU HNSW
+  baseline, not a checker of the GPU path): with a beam as wide as the base
+  the answer is the brute-force top-k over the nodes reachable on layer 0;
+  renumbering in BFS order changes no answer; its recall equals the compiled
+  reference's within 0.02 on the same data and parameters
+  (tests/test_hnsw_cpu.py).
 * ``gf_keys``, ``gf_query_reference`` (gpu_filter.cu:52-63, 142-176): the
   whole chain brute-forced in integer arithmetic on rows whose codes equal
   their values (every key and re-rank exact), duplicates for the stable
@@ -405,3 +411,68 @@ def ref_module(name):
         return importlib.import_module(name)
     except ImportError:
         return None
+
+
+class HnswCpu:
+    """The CPU HNSW baseline (oracle/hnsw_cpu.cpp, built into
+    oracle/libhnsw_cpu.so): submission/src/hnsw.cpp restated with 64-bit
+    offsets.  metric: "float", "sq8" (one global scale) or "sq8pd"."""
+
+    _L = None
+
+    @classmethod
+    def _lib(cls):
+        if cls._L is None:
+            path = os.path.join(HERE, "libhnsw_cpu.so")
+            if not os.path.exists(path):
+                build()
+            L = ctypes.CDLL(path)
+            L.hnsw_cpu_build.restype = ctypes.c_void_p
+            L.hnsw_cpu_build.argtypes = [_f32p, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, ctypes.c_uint64,
+                                         _i64]
+            L.hnsw_cpu_query.restype = _i32
+            L.hnsw_cpu_query.argtypes = [ctypes.c_void_p, _f32p, _i32, _i32, _i64p]
+            L.hnsw_cpu_query_batch.restype = _i32
+            L.hnsw_cpu_query_batch.argtypes = [ctypes.c_void_p, _f32p, _i64, _i32, _i32, _i64p, _i32]
+            L.hnsw_cpu_ndist.restype = _i64
+            L.hnsw_cpu_ndist.argtypes = [ctypes.c_void_p]
+            L.hnsw_cpu_free.restype = None
+            L.hnsw_cpu_free.argtypes = [ctypes.c_void_p]
+            cls._L = L
+        return cls._L
+
+    def __init__(self, x, M=16, ef_construction=100, metric="float", heuristic=False, reorder=False, threads=0,
+                 seed=42, progress=0):
+        L = self._lib()
+        x = _f32(x)
+        if x.ndim != 2:
+            raise RuntimeError("Input must be a non-empty 2D array")
+        mcode = {"float": 0, "sq8": 1, "sq8pd": 2}[metric]
+        self.d = x.shape[1]
+        self._h = L.hnsw_cpu_build(x, x.shape[0], x.shape[1], int(M), int(ef_construction), mcode, int(heuristic),
+                                   int(reorder), int(threads), int(seed), int(progress))
+        if not self._h:
+            raise RuntimeError("hnsw_cpu_build refused its arguments")
+
+    def query(self, q, k, ef=100):
+        out = np.empty(int(k), dtype=np.int64)
+        m = self._lib().hnsw_cpu_query(self._h, _f32(q).ravel(), int(k), int(ef), out)
+        return out[:m]
+
+    def query_batch(self, Q, k, ef=100, threads=1):
+        Q = _f32(Q)
+        out = np.empty((Q.shape[0], int(k)), dtype=np.int64)
+        self._lib().hnsw_cpu_query_batch(self._h, Q, Q.shape[0], int(k), int(ef), out, int(threads))
+        return out
+
+    @property
+    def n_distances(self):
+        return int(self._lib().hnsw_cpu_ndist(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib().hnsw_cpu_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
