@@ -35,7 +35,8 @@ __global__ __launch_bounds__(kThreads) void select_threshold(SelectArgs a) {
     __shared__ uint32_t sh[8];
     __shared__ uint32_t wsum[kThreads / 64];
     __shared__ float tsh;
-    extern __shared__ uint32_t skeys[];  // [kStageKeys]
+    extern __shared__ uint32_t skeys[];  // [kStageKeys] keys, then [kStageKeys] row ids
+    uint32_t* srids = skeys + kStageKeys;
     const int q = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t cnt = a.cand.cnt[q];
@@ -49,18 +50,26 @@ __global__ __launch_bounds__(kThreads) void select_threshold(SelectArgs a) {
     const bool staged = cnt <= (uint32_t)kStageKeys;
     uint32_t k;
     if (staged) {
+        // keys and row ids staged together (the compaction below then reads
+        // only LDS; with ids read from global memory per 256-entry round it
+        // waited on a load latency per round)
         constexpr int U = 8;
         for (int i0 = 0; i0 < (int)cnt; i0 += kThreads * U) {
             float v[U];
+            uint32_t r[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = i0 + u * kThreads + (int)threadIdx.x;
                 v[u] = i < (int)cnt ? keys[i] : 0.f;
+                r[u] = i < (int)cnt ? rids[i] : 0u;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = i0 + u * kThreads + (int)threadIdx.x;
-                if (i < (int)cnt) skeys[i] = f2key(v[u]);
+                if (i < (int)cnt) {
+                    skeys[i] = f2key(v[u]);
+                    srids[i] = r[u];
+                }
             }
         }
         __syncthreads();
@@ -81,6 +90,46 @@ __global__ __launch_bounds__(kThreads) void select_threshold(SelectArgs a) {
     // writes, and a kept entry moves to an index no larger than its own
     const float t = tsh;
     uint32_t base = 0;
+    if (staged) {
+        // from LDS, four consecutive entries a thread per round
+        constexpr int V = 4;
+        for (int i0 = 0; i0 < (int)cnt; i0 += kThreads * V) {
+            const int i1 = i0 + (int)threadIdx.x * V;
+            bool keep[V];
+            uint32_t nk = 0;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                keep[v] = i1 + v < (int)cnt && key2f(skeys[i1 + v]) <= t;
+                nk += keep[v] ? 1u : 0u;
+            }
+            // exclusive prefix of nk over the block: in the wave, then the waves
+            uint32_t incl = nk;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t u = __shfl_up(incl, o);
+                if (lane >= o) incl += u;
+            }
+            if (lane == 63) wsum[wave] = incl;
+            __syncthreads();
+            uint32_t off = base + incl - nk, total = 0;
+#pragma unroll
+            for (int w = 0; w < kThreads / 64; ++w) {
+                if (w < wave) off += wsum[w];
+                total += wsum[w];
+            }
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (keep[v]) {
+                    keys[off] = key2f(skeys[i1 + v]);
+                    rids[off] = srids[i1 + v];
+                    ++off;
+                }
+            base += total;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.cand.cnt[q] = base;
+        return;
+    }
     for (int i0 = 0; i0 < (int)cnt; i0 += kThreads) {
         const int i = i0 + (int)threadIdx.x;
         float kf = 0.f;
@@ -277,7 +326,7 @@ __global__ __launch_bounds__(kThreads) void finalize(FinalArgs a) {
 
 hipError_t launch_select_threshold(const SelectArgs& a, hipStream_t st) {
     if (a.B <= 0) return hipSuccess;
-    ANNMI_LAUNCH(select_threshold, dim3((unsigned)a.B), dim3(kThreads), kStageKeys * sizeof(uint32_t), st, a);
+    ANNMI_LAUNCH(select_threshold, dim3((unsigned)a.B), dim3(kThreads), 2 * kStageKeys * sizeof(uint32_t), st, a);
     return hipGetLastError();
 }
 
