@@ -439,6 +439,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (open-loop) leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the one-query exact leg on 1M x 384")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 top-400 leg of the default workload")
     ap.add_argument("--hnsw-rows", type=int, default=200_000,
                     help="rows of the base the CPU HNSW baseline is built on (10000000: the whole headline base, "
                          "a build of several minutes)")
@@ -564,6 +565,32 @@ def main():
                      "scan_ms": round(pprof["scan_ms"], 2)}
             log(f"[bench] plain fp32: {plain}")
 
+    # the precision variant of the same workload: BF16 MFMA candidates with a
+    # top-400 fp32 re-score (and the fp64 near-tie step), on the same timed
+    # batches, recall against the exact scan
+    bf16 = None
+    if not args.no_bf16 and mode == "sq8" and args.fixed_batch == 0:
+        bx = Index(base, mode="bf16", rerank=400)
+        if not args.no_autotune:
+            bx.autotune(max_batch=4096, reps=1)
+        run_batches(bx, Q[: int(wsz.sum())], wsz, k, 400)
+        torch.cuda.synchronize()
+        tb0 = time.perf_counter()
+        btimes, bids, _ = run_batches(bx, Qt, tsz, k, 400)
+        torch.cuda.synchronize()
+        bwall = time.perf_counter() - tb0
+        bpred = torch.cat(bids)
+        gtb, _ = exact.search_device(Qt[sel], k)
+        brec = float(recall_vs_exact(base, Qt[sel], bpred[sel], gtb, k).mean())
+        blat = np.repeat(btimes, tsz) * 1e3
+        bf16 = {"mode": "bf16", "rerank": 400, "qps": round(nq / bwall, 1), "recall_at_100": round(brec, 5),
+                "p50_ms": round(float(np.percentile(blat, 50)), 3), "p99_ms": round(float(np.percentile(blat, 99)), 3),
+                "speedup_vs_plain_fp32": round(nq / bwall / plain["qps"], 3) if plain else None}
+        bx.close()
+        del bx
+        torch.cuda.empty_cache()
+        log(f"[bench] bf16 R=400: {bf16}")
+
     # served leg: the same index behind serving.Server, requests arriving as
     # a Poisson process with sizes log-uniform in [1, 4096], latency from
     # arrival to completion of each request
@@ -655,6 +682,7 @@ def main():
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "served": served,
         "one_query_exact_1m": c2,
+        "bf16_r400": bf16,
         "plain_fp32": plain,
         "speedup_vs_plain_fp32": round(qps / plain["qps"], 3) if plain else None,
         "roofline": roof,
