@@ -21,7 +21,8 @@
 //                         candidate closer to the base than to every kept
 //                         neighbour, by exact fp32 distances; back-fill with
 //                         the nearest discarded (keepPrunedConnections)
-//   prune     :570-582    re-select a node's list when it exceeds M_max
+//   prune     :570-582    a full list and a new link re-selected by exact
+//                         fp32 distances to the node
 //   insert    :587-645    descent with ef 1, then per layer a beam of
 //                         ef_construction, M (2M on layer 0) neighbours,
 //                         links both ways under per-node locks
@@ -257,20 +258,6 @@ struct Graph {
         for (size_t i = 0; i < dropped.size() && (int)out.size() < mmax; ++i) out.push_back(dropped[i]);
     }
 
-    // re-select node c's list at a layer (caller holds c's lock)
-    void prune(int64_t c, int layer) {
-        int32_t* l = list(c, layer);
-        const int mmax = cap(layer);
-        std::vector<DistId> cand;
-        cand.reserve((size_t)l[0]);
-        for (int t = 0; t < l[0]; ++t) cand.push_back({l2_f32(row(l[1 + t]), row(c), d, kInf), l[1 + t]});
-        std::sort(cand.begin(), cand.end());
-        std::vector<int32_t> keep;
-        select(cand, mmax, keep);
-        l[0] = (int32_t)keep.size();
-        std::memcpy(l + 1, keep.data(), sizeof(int32_t) * keep.size());
-    }
-
     void shift(const float* q, std::vector<float>& qs) const {
         if (metric != kSq8pd) return;
         qs.resize((size_t)d);
@@ -308,7 +295,9 @@ struct Graph {
                     ln[1 + ln[0]] = (int32_t)idx;
                     ++ln[0];
                 } else {
-                    // full: the new link joins the candidates of a re-selection
+                    // full: the list and the new link are re-selected (the
+                    // reference appends, then prunes back to the cap,
+                    // hnsw.cpp:637-640, 570-582)
                     std::vector<DistId> cand;
                     cand.reserve((size_t)ln[0] + 1);
                     const float* pn = row(nb);
