@@ -186,7 +186,7 @@ __device__ inline i32x4 load_row_frag(const uint8_t* X, int64_t xstride, int64_t
 // Query tile in LDS: 16*QW queries x kdim elements, stored in the scan
 // element type, row stride padded by 16 bytes against bank conflicts.
 template <class P>
-__host__ __device__ inline int qrow_bytes(int kdim) {
+__host__ __device__ constexpr int qrow_bytes(int kdim) {
     int b = P::kElemBytes == 0 ? kdim : kdim * P::kElemBytes;  // SQ4 query weights are int8 per dim
     return b + 16;
 }
@@ -662,7 +662,9 @@ __device__ inline int wide_row_offset(const ScanArgs& a, float xn, float kap) {
     return v > 0.f ? (v < 16777216.f ? (int)v : 16777216) : 0;
 }
 
-template <class P, int QS, int RS, int kWideWaves, int QH, bool kQPre>
+// KD > 0: the padded dim, fixed at compile time (query-tile offsets become
+// instruction offsets and the chunk loop unrolls whole); 0: a.kdim.
+template <class P, int QS, int RS, int kWideWaves, int QH, bool kQPre, int KD>
 __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, int qtiles, int64_t chunk_rows,
                                                                     int64_t nrc, int qcap) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -688,8 +690,9 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     const int qbase = 16 * QS * (wave % QH);
     const int stream = wave / QH;
     const int q0 = qtile * QT;
-    const int qbytes = P::kElemBytes == 0 ? a.kdim : a.kdim * P::kElemBytes;
-    const int ldq = qrow_bytes<P>(a.kdim);
+    const int kdim = KD > 0 ? KD : a.kdim;
+    const int qbytes = P::kElemBytes == 0 ? kdim : kdim * P::kElemBytes;
+    const int ldq = qrow_bytes<P>(kdim);
     typedef typename P::El El;
     const WideLds L = wide_stage<El>(a, smem, QT, ldq, qbytes, q0);
     const float* thr_s = L.thr;
@@ -707,7 +710,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     const float kap = kIntDot ? wide_kap(a, L, QT, lane) : 0.f;
 
     const uint8_t* X = static_cast<const uint8_t*>(a.X);
-    const int nchunk = a.kdim / P::kChunk;
+    const int nchunk = kdim / P::kChunk;
     const int lane_dim = P::kDimsPerLoad * g;
     constexpr int TR = 16 * RS;  // rows per wave step
     constexpr int D = P::kRing;
@@ -831,6 +834,9 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                     i32x4 bb[RS];
 #pragma unroll
                     for (int r = 0; r < RS; ++r) bb[r] = to_frag<P>(ring[s2][r]);
+                    // the other wave of the SIMD yields while these MFMAs
+                    // issue (10M x 384 SQ8, 1024 queries: 1760 -> 1846 TOP/s)
+                    if constexpr (KD > 0) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
                     for (int q = 0; q < QS; ++q)
 #pragma unroll
@@ -845,6 +851,16 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                             } else
                                 P::mma(acc[q][r], qa[s2 & 1][q], bb[r]);
                         }
+                    if constexpr (KD > 0) __builtin_amdgcn_s_setprio(0);
+                    // the slot's query fragments are read together, ahead of
+                    // its MFMAs: left to itself the scheduler reuses one
+                    // register set and waits out each read's latency before
+                    // every second MFMA (10M x 384 SQ8, 4096 queries: 1804 ->
+                    // 1943 TOP/s; with the dim fixed at compile time, 1962)
+                    if constexpr (!kQPre) {
+                        __builtin_amdgcn_sched_group_barrier(0x100, QS, 0);
+                        __builtin_amdgcn_sched_group_barrier(0x008, QS * RS, 0);
+                    }
                 }
                 const int cn = c + D;
                 int cc = same ? cn : cn - npad;
@@ -938,13 +954,14 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                 }
             }
         }
-        // keep room for a tile's typical appends
-        if (__builtin_amdgcn_readfirstlane(*wq.count) + 512u > (uint32_t)qcap) wide_flush(a, wq, q0, r_begin, lane);
+        // keep room for a tile's typical appends (half of a short queue)
+        if (__builtin_amdgcn_readfirstlane(*wq.count) + (qcap >= 1024 ? 512u : (uint32_t)qcap / 2) > (uint32_t)qcap)
+            wide_flush(a, wq, q0, r_begin, lane);
     }
     wide_flush(a, wq, q0, r_begin, lane);
 }
 
-template <class P, int QS, int RS, int NW, int QH, bool QPRE>
+template <class P, int QS, int RS, int NW, int QH, bool QPRE, int KD = 0>
 hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     static const char* const name = [] {
         static char buf[80];
@@ -957,7 +974,7 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     constexpr int QT = 16 * QS * QH;
     const size_t lds = (size_t)QT * qrow_bytes<P>(a.kdim) + 5 * QT * sizeof(float) + NW * sizeof(uint32_t);
     static const bool attr = [] {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide_kernel<P, QS, RS, NW, QH, QPRE>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_wide_kernel<P, QS, RS, NW, QH, QPRE, KD>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         return true;
     }();
@@ -994,8 +1011,9 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
         const int v = atoi(e);
         if (v >= 0 && v < qcap) qcap = v / 64 * 64;
     }
-    if (qcap < 1024) qcap = 0;
-    ANNMI_LAUNCH((scan_wide_kernel<P, QS, RS, NW, QH, QPRE>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
+    if (qcap < 64) qcap = 0;
+    if (KD > 0 && a.kdim != KD) return hipErrorInvalidValue;
+    ANNMI_LAUNCH((scan_wide_kernel<P, QS, RS, NW, QH, QPRE, KD>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
                        lds + (size_t)NW * qcap * 8 + hist_bytes, st, a, qtiles, chunk, nrc, qcap);
     return hipGetLastError();
 }
@@ -2060,8 +2078,14 @@ hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
         // fp32 (small d): 128 x 64 tiles on 4 waves; bf16 and the byte codes:
         // 128 x 32 tiles on 8 waves (with the tiled row layout bf16 measured
         // 1053 vs 952 TFLOP/s at 4096 queries, 10M x 384)
-        if constexpr (P::kElemBytes == 4) return launch_wide_t<P, 8, 4, 4, 1, true>(a, st);
-        else return launch_wide_t<P, 8, 2, 8, 1, false>(a, st);
+        if constexpr (P::kElemBytes == 4) {
+            return launch_wide_t<P, 8, 4, 4, 1, true>(a, st);
+        } else {
+            // 384-dim SQ8 with the dim fixed at compile time
+            if constexpr (std::is_same<P, PolSQ8>::value)
+                if (a.kdim == 384) return launch_wide_t<P, 8, 2, 8, 1, false, 384>(a, st);
+            return launch_wide_t<P, 8, 2, 8, 1, false>(a, st);
+        }
     }
     if (64 * row <= 160 * 1024) return launch_wide_t<P, 4, 4, 8, 1, false>(a, st);
     if (32 * row <= 160 * 1024) return launch_wide_t<P, 2, 4, 8, 1, false>(a, st);

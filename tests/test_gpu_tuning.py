@@ -62,3 +62,26 @@ def test_autotune_table_keeps_answers(inputs):
         np.testing.assert_array_equal(dd, ref_d[:B])
     with pytest.raises(RuntimeError):
         idx.set_scan_shape(1, 9, 0)  # past the last configuration
+
+
+@pytest.mark.parametrize("B", [1, 3, 300])
+def test_sq8_384_specialisation_equals_the_runtime_dim_kernel(B):
+    """SQ8 at d = 384 runs a wide-scan instantiation with the dim fixed at
+    compile time; the same tile shape with the dim read at run time (wide
+    configuration 2) gives bit-identical answers, on a row count that ends
+    mid-tile, with the built-in and a small row chunk."""
+    n = 150_003
+    print(f"[case] sq8 d=384 specialisation B={B}", flush=True)
+    base = data.mixture(n, 384, 512, 0.3, 64, center_seed=65)
+    Q = data.mixture(B, 384, 512, 0.3, 66, center_seed=65)
+    idx = Index(base, mode="sq8", rerank=200)
+    for ch in (0, 8192):
+        idx.set_scan_shape(1, 0, ch)
+        ref_ids, ref_d = idx.search(Q, 100)
+        idx.set_scan_shape(1, 2, ch)
+        ids, dd = idx.search(Q, 100)
+        np.testing.assert_array_equal(ids, ref_ids, err_msg=f"chunk {ch}")
+        np.testing.assert_array_equal(dd, ref_d, err_msg=f"chunk {ch}")
+    o_ids, _ = O.exact_topk_f64(base, Q[:2], 100)
+    for i in range(min(B, 2)):
+        assert len(set(ref_ids[i]) & set(o_ids[i])) >= 99
