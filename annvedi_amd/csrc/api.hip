@@ -490,7 +490,10 @@ int range_bounds(int64_t n, int rsel, int cap, int B, int64_t* b, int maxb) {
     static const int64_t env_s0 = getenv("ANNMI_RANGE_S0") ? atoll(getenv("ANNMI_RANGE_S0")) : 0;
     static const int64_t env_rho = getenv("ANNMI_RANGE_RHO") ? atoll(getenv("ANNMI_RANGE_RHO")) : 0;
     static const int64_t env_b = getenv("ANNMI_RANGE_BATCH") ? atoll(getenv("ANNMI_RANGE_BATCH")) : 0;
-    const bool large = B >= (env_b > 0 ? env_b : 16);
+    // (10M x 384 SQ8 a step: 16 queries 0.948 -> 0.844 ms and 256 queries
+    // 1.497 -> 1.382 ms with the small-batch rule; 512 queries 2.483 ->
+    // 2.331 ms with the large-batch one)
+    const bool large = B >= (env_b > 0 ? env_b : 384);
     int nb = 0;
     b[nb++] = 0;
     int64_t s0 = env_s0 > 0 ? env_s0 : (large ? std::max<int64_t>(1024, 8 * (int64_t)rsel)

@@ -740,9 +740,22 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     // holding the chunk's first row).
     const int64_t cstep = a.tiled ? 16 * CB : CB;
     const uint8_t* xbase = a.tiled ? X + (r_begin >> 4) * 16 * a.xstride : X + r_begin * a.xstride;
+    // KD > 0 (tiled rows only): rows in 32-bit arithmetic relative to the
+    // chunk's first row; u = (r_begin % 16) + row counts rows from the tile
+    // holding r_begin
+    const uint32_t nrows32 = (uint32_t)(r_end - r_begin);
+    const uint32_t rb15 = (uint32_t)(r_begin & 15);
+    const uint32_t t16 = (uint32_t)(16 * a.xstride);
     auto tile_offs = [&](int64_t t, uint32_t (&o)[RS]) {
 #pragma unroll
         for (int r = 0; r < RS; ++r) {
+            if constexpr (KD > 0) {
+                uint32_t row = (uint32_t)t * TR + 16 * r + n;
+                row = row < nrows32 ? row : nrows32 - 1;
+                const uint32_t u = rb15 + row;
+                o[r] = (u >> 4) * t16 + (u & 15) * CB + g * lane_bytes<P>();
+                continue;
+            }
             int64_t row = t * TR + 16 * r + n;
             if (row >= r_end - r_begin) row = r_end - r_begin - 1;
             if (a.tiled) {
@@ -757,6 +770,26 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     auto qfrag_off = [&](int c) {
         return P::kElemBytes == 0 ? (c * P::kChunk + lane_dim) : (c * P::kChunk + lane_dim) * P::kElemBytes;
     };
+    // row norms, loaded a tile ahead: the first MFMAs of a tile take them
+    // (prefilter 3) without waiting for that tile's chunk prefetches.  The
+    // first tile's are requested before its chunks, as every later tile's
+    // are (a tile ahead): the wait for them at a tile's start then leaves
+    // the chunk loads in flight on every pass of the loop, the first one
+    // included (the compiler's wait covers all passes alike).
+    float xnn[RS];
+    auto load_xn = [&](int64_t t) {
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            if constexpr (KD > 0) {
+                const uint32_t row = (uint32_t)t * TR + 16 * r + n;
+                xnn[r] = (a.xn + r_begin)[row < nrows32 ? row : nrows32 - 1];
+            } else {
+                const int64_t row = r_begin + t * TR + 16 * r + n;
+                xnn[r] = a.xn[row < r_end ? row : r_end - 1];
+            }
+        }
+    };
+    load_xn(stream);
     tile_offs(stream, cur);
     // issued in slot order (the scheduler may not reorder them: the loop's
     // waits assume slot s was requested before slot s + 1)
@@ -774,27 +807,16 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
             qa[0][q] = *reinterpret_cast<const i32x4*>(smem + (qbase + 16 * q + n) * ldq + qfrag_off(0));
     }
 
-    // row norms, loaded a tile ahead: the first MFMAs of a tile take them
-    // (prefilter 3) without waiting for that tile's chunk prefetches
-    float xnn[RS];
-    auto load_xn = [&](int64_t t) {
-#pragma unroll
-        for (int r = 0; r < RS; ++r) {
-            const int64_t row = r_begin + t * TR + 16 * r + n;
-            xnn[r] = a.xn[row < r_end ? row : r_end - 1];
-        }
-    };
-    load_xn(stream);
 
     for (int64_t t = stream; t < ntiles; t += NS) {
         const int64_t r0 = r_begin + t * TR;
-        tile_offs(t, cur);
+        if constexpr (KD == 0) tile_offs(t, cur);  // KD > 0: the previous tile's nxt
         tile_offs(t + NS < ntiles ? t + NS : t, nxt);
         bool rok[RS];
         float xn[RS];
 #pragma unroll
         for (int r = 0; r < RS; ++r) {
-            rok[r] = r0 + 16 * r + n < r_end;
+            rok[r] = KD > 0 ? (uint32_t)t * TR + 16 * r + n < nrows32 : r0 + 16 * r + n < r_end;
             xn[r] = xnn[r];
         }
         load_xn(t + NS < ntiles ? t + NS : t);
@@ -957,6 +979,10 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
         // keep room for a tile's typical appends (half of a short queue)
         if (__builtin_amdgcn_readfirstlane(*wq.count) + (qcap >= 1024 ? 512u : (uint32_t)qcap / 2) > (uint32_t)qcap)
             wide_flush(a, wq, q0, r_begin, lane);
+        if constexpr (KD > 0) {
+#pragma unroll
+            for (int r = 0; r < RS; ++r) cur[r] = nxt[r];
+        }
     }
     wide_flush(a, wq, q0, r_begin, lane);
 }
@@ -996,14 +1022,9 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
     // over 10M x 384 SQ8: 1.166 -> 1.084 ms of scan a step with one per CU;
     // 512 queries best at 4, 4096 at 4-8)
     const int64_t wg_per_cu = qtiles < 8 ? qtiles : 8;
-    int64_t chunk = rows * qtiles / ((int64_t)ncu * wg_per_cu);
-    chunk = chunk < step ? step : (chunk > chunk_max ? chunk_max : chunk);
-    chunk = (chunk + step - 1) / step * step;
-    const int64_t nrc = (rows + chunk - 1) / chunk;
-    const int64_t per = (nrc + 7) / 8;
     // the rest of the LDS holds the waves' flush histograms and candidate
-    // queues (8 bytes an entry); below 1024 entries a wave, every pair goes
-    // straight to the lists
+    // queues (8 bytes an entry); a short queue flushes at half full, none
+    // sends every pair straight to the lists
     const size_t hist_bytes = (size_t)NW * kFlushSlots * sizeof(uint32_t);
     int qcap = (int)((160 * 1024 - lds - hist_bytes) / (NW * 8)) / 64 * 64;
     if (qcap > 4096) qcap = 4096;
@@ -1012,9 +1033,15 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
         if (v >= 0 && v < qcap) qcap = v / 64 * 64;
     }
     if (qcap < 64) qcap = 0;
-    if (KD > 0 && a.kdim != KD) return hipErrorInvalidValue;
+    const size_t lds_total = lds + (size_t)NW * qcap * 8 + hist_bytes;
+    int64_t chunk = rows * qtiles / ((int64_t)ncu * wg_per_cu);
+    chunk = chunk < step ? step : (chunk > chunk_max ? chunk_max : chunk);
+    chunk = (chunk + step - 1) / step * step;
+    const int64_t nrc = (rows + chunk - 1) / chunk;
+    const int64_t per = (nrc + 7) / 8;
+    if (KD > 0 && (a.kdim != KD || !a.tiled)) return hipErrorInvalidValue;
     ANNMI_LAUNCH((scan_wide_kernel<P, QS, RS, NW, QH, QPRE, KD>), dim3((unsigned)(8 * per * qtiles)), dim3(64 * NW),
-                       lds + (size_t)NW * qcap * 8 + hist_bytes, st, a, qtiles, chunk, nrc, qcap);
+                       lds_total, st, a, qtiles, chunk, nrc, qcap);
     return hipGetLastError();
 }
 
@@ -2083,7 +2110,7 @@ hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
         } else {
             // 384-dim SQ8 with the dim fixed at compile time
             if constexpr (std::is_same<P, PolSQ8>::value)
-                if (a.kdim == 384) return launch_wide_t<P, 8, 2, 8, 1, false, 384>(a, st);
+                if (a.kdim == 384 && a.tiled) return launch_wide_t<P, 8, 2, 8, 1, false, 384>(a, st);
             return launch_wide_t<P, 8, 2, 8, 1, false>(a, st);
         }
     }
