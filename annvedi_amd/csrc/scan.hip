@@ -1426,6 +1426,14 @@ hipError_t launch_wide32_t(const ScanArgs& a, hipStream_t st) {
 // (20.2 vs 18.0 ms of scan).  One wave a SIMD (RS = 2 or 4 on 4 waves, the
 // accumulators in AGPRs) measured 1470 / 855 TOP/s: a single wave does not
 // hide the row-load latency with a ring of 4 K steps.
+// LDS the 32 x 32 lean-epilogue kernel needs at the least: the 128-query
+// tile, five words a query, the queue counters and flush histograms, a 4 KB
+// exact-test slot a wave and a queue of 1024 entries a wave
+constexpr size_t i8x32_lds_need(int qbytes, int nw) {
+    return (size_t)128 * (qbytes + 16) + 5 * 128 * sizeof(float) + nw * sizeof(uint32_t) +
+           (size_t)nw * kFlushSlots * sizeof(uint32_t) + (size_t)nw * 4096 + (size_t)nw * 1024 * 8;
+}
+
 template <int RS, int D, int NW>
 __global__ __launch_bounds__(64 * NW) void scan_i8x32_kernel(ScanArgs a, int qtiles, int64_t chunk_rows, int64_t nrc,
                                                               int qcap) {
@@ -1618,7 +1626,7 @@ hipError_t launch_i8x32_t(const ScanArgs& a, hipStream_t st) {
     const size_t lds = (size_t)QT * (qbytes + 16) + 5 * QT * sizeof(float) + NW * sizeof(uint32_t);
     const size_t hist_bytes = (size_t)NW * kFlushSlots * sizeof(uint32_t);
     const size_t stage_bytes = (size_t)NW * 4096;
-    if (lds + hist_bytes + stage_bytes + (size_t)NW * 1024 * 8 > 160 * 1024) return hipErrorInvalidValue;
+    if (i8x32_lds_need(qbytes, NW) > 160 * 1024) return hipErrorInvalidValue;
     static const bool attr = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_i8x32_kernel<RS, D, NW>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -2175,7 +2183,12 @@ bool wide_cfg_fits(int kind, int kdim, int cfg) {
         cfg == kWideCfg32R2W4) {  // 128 queries, whole 64-byte row chunks
         if (kind != SCAN_SQ8 && kind != SCAN_BF16) return false;
         const int qbytes = kind == SCAN_SQ8 ? kdim : 2 * kdim;
-        return qbytes % 64 == 0 && (size_t)128 * (qbytes + 16 + 20) + 8 * 1024 <= 160 * 1024;
+        if (qbytes % 64 != 0 || (size_t)128 * (qbytes + 16 + 20) + 8 * 1024 > 160 * 1024) return false;
+        // SQ8 on these two takes the lean-epilogue kernel, whose fixed LDS
+        // (exact-test slots, a minimum queue) leaves room for fewer dims
+        if (kind == SCAN_SQ8 && (cfg == kWideCfg32R4W4 || cfg == kWideCfg32R2W4))
+            return i8x32_lds_need(qbytes, 8) <= 160 * 1024;
+        return true;
     }
     switch (kind) {
         case SCAN_F32: return wide_fits<PolF32>(qs, kdim);

@@ -85,3 +85,32 @@ def test_sq8_384_specialisation_equals_the_runtime_dim_kernel(B):
     o_ids, _ = O.exact_topk_f64(base, Q[:2], 100)
     for i in range(min(B, 2)):
         assert len(set(ref_ids[i]) & set(o_ids[i])) >= 99
+
+
+def test_sq8_768_every_shape_fits_or_is_refused():
+    """At d = 768 the 32 x 32 lean-epilogue shapes need more LDS than a
+    workgroup has: setting them is refused up front, every other shape gives
+    the built-in answers, and the autotuner (which tries every shape that
+    fits) completes."""
+    print("[case] sq8 d=768 shapes and autotune", flush=True)
+    base = data.mixture(30_000, 768, 64, 0.3, 67, center_seed=68)
+    Q = data.mixture(200, 768, 64, 0.3, 69, center_seed=68)
+    idx = Index(base, mode="sq8", rerank=300)
+    ref_ids, ref_d = idx.search(Q, 100)
+    refused = []
+    for cfg in range(1, 9):
+        try:
+            idx.set_scan_shape(1, cfg, 0)
+        except RuntimeError as e:
+            assert "does not fit" in str(e) or "bad scan shape" in str(e)
+            refused.append(cfg)
+            continue
+        ids, dd = idx.search(Q, 100)
+        np.testing.assert_array_equal(ids, ref_ids, err_msg=f"cfg {cfg}")
+        np.testing.assert_array_equal(dd, ref_d, err_msg=f"cfg {cfg}")
+    assert 6 in refused and 7 in refused and 8 in refused, refused
+    idx.set_scan_shape(0, 0, 0)
+    table = idx.autotune(max_batch=256, reps=1)
+    assert len(table) > 0
+    ids, dd = idx.search(Q, 100)
+    np.testing.assert_array_equal(ids, ref_ids)
