@@ -95,6 +95,32 @@ struct PolFP8 {
     }
 };
 
+// FP8 on the gfx950 16x16x128 f8f6f4 MFMA (e4m3 operands, unit scales),
+// which issues at twice the rate of 16x16x32 fp8 (MI355X, register
+// operands: 4.98 vs 2.32 PFLOP/s): the wide scan pairs consecutive 64-dim
+// chunks, a lane's 16 bytes of each making its 32 bytes of one 128-deep K
+// step.  Queries and rows take the same order, which is all a dot product
+// needs.  The tiled layout and the fragments are those of PolFP8.
+struct PolFP8P : PolFP8 {
+    __device__ static inline void mma8(v4f& acc, const i32x8& a, const i32x8& b) {
+        // formats 0 / 0: fp8 e4m3 for both operands; scale exponents 127: 2^0
+        acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc, 0, 0, 0, 127, 0, 127);
+    }
+};
+// half h of an 8-register operand (the pair's chunk c - 1 or c), written in
+// place so the MFMA takes the tuple without copies
+__device__ __forceinline__ void set_half(i32x8& v, int h, const i32x4& x) {
+    if (h) {
+        v[4] = x[0], v[5] = x[1], v[6] = x[2], v[7] = x[3];
+    } else {
+        v[0] = x[0], v[1] = x[1], v[2] = x[2], v[3] = x[3];
+    }
+}
+template <class P>
+struct IsPair : std::false_type {};
+template <>
+struct IsPair<PolFP8P> : std::true_type {};
+
 // int8 MFMA with an int32 accumulator.
 struct PolSQ8 {
     static constexpr int kDimsPerLoad = 16;
@@ -406,6 +432,7 @@ const char* pol_name();
 template <> const char* pol_name<PolF32>() { return "f32"; }
 template <> const char* pol_name<PolBF16>() { return "bf16"; }
 template <> const char* pol_name<PolFP8>() { return "fp8"; }
+template <> const char* pol_name<PolFP8P>() { return "fp8x128"; }
 template <> const char* pol_name<PolSQ8>() { return "sq8"; }
 template <> const char* pol_name<PolSQ4>() { return "sq4"; }
 
@@ -707,6 +734,8 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     wq.hist = L.cnt + kWideWaves + (int64_t)kWideWaves * 2 * qcap + wave * kFlushSlots;
     wq.cap = qcap;
     constexpr bool kIntDot = std::is_same<El, int>::value;
+    constexpr bool kPair = IsPair<P>::value;
+    static_assert(!kPair || (!kQPre && P::kRing % 2 == 0), "pairs: in-slot query reads, an even ring");
     const float kap = kIntDot ? wide_kap(a, L, QT, lane) : 0.f;
 
     const uint8_t* X = static_cast<const uint8_t*>(a.X);
@@ -734,6 +763,8 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     // (a uniform base, so the offset is the only per-lane operand).
     typedef typename RawOf<P>::T Raw;
     Raw ring[D][RS];
+    i32x8 rp[D / 2][RS];  // pairs: the ring as 8-register operands (slot s in half s % 2 of rp[s / 2])
+    i32x8 qa8[QS];        // pairs: query fragments of chunks c - 1 and c
     uint32_t cur[RS], nxt[RS];
     // Tiled layout: chunk c of a 16-row tile lies 16 * CB bytes after chunk
     // c - 1, and lane n's row is the tile's row n (row offsets from the tile
@@ -797,7 +828,12 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
     for (int s2 = 0; s2 < D; ++s2) {
         const uint8_t* xb = xbase + (int64_t)(s2 < nchunk ? s2 : nchunk - 1) * cstep;
 #pragma unroll
-        for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>(xb + cur[r]);
+        for (int r = 0; r < RS; ++r) {
+            if constexpr (kPair)
+                set_half(rp[s2 >> 1][r], s2 & 1, load_raw<P>(xb + cur[r]));
+            else
+                ring[s2][r] = load_raw<P>(xb + cur[r]);
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
     i32x4 qa[2][QS];
@@ -847,12 +883,31 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                             *reinterpret_cast<const i32x4*>(smem + (qbase + 16 * q + n) * ldq + qfrag_off(cq));
                 }
                 if (c < nchunk) {
-                    if constexpr (!kQPre) {
+                    if constexpr (kPair) {
+#pragma unroll
+                        for (int q = 0; q < QS; ++q)
+                            set_half(qa8[q], s2 & 1,
+                                     *reinterpret_cast<const i32x4*>(smem + (qbase + 16 * q + n) * ldq + qfrag_off(c)));
+                    } else if constexpr (!kQPre) {
 #pragma unroll
                         for (int q = 0; q < QS; ++q)
                             qa[s2 & 1][q] =
                                 *reinterpret_cast<const i32x4*>(smem + (qbase + 16 * q + n) * ldq + qfrag_off(c));
                     }
+                    if constexpr (kPair) {
+                        // chunks (c - 1, c) in one 128-deep MFMA at odd slots
+                        // (D is even and turns start at multiples of D, so a
+                        // pair never straddles a turn; nchunk is even)
+                        if (s2 & 1) {
+#pragma unroll
+                            for (int q = 0; q < QS; ++q)
+#pragma unroll
+                                for (int r = 0; r < RS; ++r) {
+                                    if (kFirst && s2 == 1) acc[q][r] = (typename P::Acc){0.f, 0.f, 0.f, 0.f};
+                                    P::mma8(acc[q][r], qa8[q], rp[s2 >> 1][r]);
+                                }
+                        }
+                    } else {
                     i32x4 bb[RS];
 #pragma unroll
                     for (int r = 0; r < RS; ++r) bb[r] = to_frag<P>(ring[s2][r]);
@@ -874,6 +929,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                                 P::mma(acc[q][r], qa[s2 & 1][q], bb[r]);
                         }
                     if constexpr (KD > 0) __builtin_amdgcn_s_setprio(0);
+                    }
                     // the slot's query fragments are read together, ahead of
                     // its MFMAs: left to itself the scheduler reuses one
                     // register set and waits out each read's latency before
@@ -881,15 +937,24 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
                     // 1943 TOP/s; with the dim fixed at compile time, 1962)
                     if constexpr (!kQPre) {
                         __builtin_amdgcn_sched_group_barrier(0x100, QS, 0);
-                        __builtin_amdgcn_sched_group_barrier(0x008, QS * RS, 0);
+                        if (!kPair || (s2 & 1)) __builtin_amdgcn_sched_group_barrier(0x008, QS * RS, 0);
                     }
                 }
-                const int cn = c + D;
-                int cc = same ? cn : cn - npad;
-                cc = cc < nchunk ? cc : nchunk - 1;
-                const uint8_t* xb = xbase + (int64_t)cc * cstep;
+                // refill: a pair's first slot waits for the pair's MFMAs
 #pragma unroll
-                for (int r = 0; r < RS; ++r) ring[s2][r] = load_raw<P>(xb + (same ? cur[r] : nxt[r]));
+                for (int s3 = kPair ? (s2 & 1 ? s2 - 1 : D) : s2; s3 <= s2; ++s3) {
+                    const int cn = c0 + s3 + D;
+                    int cc = same ? cn : cn - npad;
+                    cc = cc < nchunk ? cc : nchunk - 1;
+                    const uint8_t* xb = xbase + (int64_t)cc * cstep;
+#pragma unroll
+                    for (int r = 0; r < RS; ++r) {
+                        if constexpr (kPair)
+                            set_half(rp[s3 >> 1][r], s3 & 1, load_raw<P>(xb + (same ? cur[r] : nxt[r])));
+                        else
+                            ring[s3][r] = load_raw<P>(xb + (same ? cur[r] : nxt[r]));
+                    }
+                }
             }
         };
         turn(0, std::true_type{});
@@ -1048,15 +1113,17 @@ hipError_t launch_wide_t(const ScanArgs& a, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Wide scan on 32 x 32 MFMA tiles (SQ8: v_mfma_i32_32x32x32_i8, BF16:
 // v_mfma_f32_32x32x16_bf16), an autotuner candidate.  One 32 x 32 MFMA does
-// twice the work of a 16 x 16 one from the same operand bytes in the same
-// issue time (measured on MI355X with register operands only: int8 4.9
-// POP/s against 2.8 for 16x16x64; bf16 16x16x32 1.44 PFLOP/s), so the
-// 16 x 16 kernel above cannot pass ~57% of the int8 and bf16 peaks.  This
-// kernel is bound before that by its LDS and load stream (SQ8 10M x 384 at
-// 4096 queries: 1.8 POP/s with everything, 2.7 without the epilogue and the
-// row loads), so it is not the default.  Same workgroup plan, query
-// staging, prefilters and queues; per wave step a 32 * RS row tile against
-// the 32 * QS query tile.
+// twice the work of a 16 x 16 one from the same operand bytes, so a query
+// fragment read from LDS feeds twice the MFMA work.  (With register
+// operands and eight independent accumulators a wave, the 16 x 16 forms
+// issue at the full rate on MI355X: int8 16x16x64 4.46 POP/s, bf16 16x16x32
+// 2.41 PFLOP/s; an earlier probe with fewer accumulators had shown 2.8 POP/s
+// and misread it as a ceiling.)  This kernel measured no faster than the
+// 16 x 16 one: bound by its LDS and load stream (SQ8 10M x 384 at 4096
+// queries: 1.8 POP/s with everything, 2.7 without the epilogue and the row
+// loads), so it is not the default.  Same workgroup plan, query staging,
+// prefilters and queues; per wave step a 32 * RS row tile against the
+// 32 * QS query tile.
 //
 // Operands: lane l holds index l % 32 (query of A, row of B) over bytes
 // [16 * (l / 32), +16) of a 32-byte K step (the same K order for A and B,
@@ -2119,6 +2186,11 @@ hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
             // 384-dim SQ8 with the dim fixed at compile time
             if constexpr (std::is_same<P, PolSQ8>::value)
                 if (a.kdim == 384 && a.tiled) return launch_wide_t<P, 8, 2, 8, 1, false, 384>(a, st);
+            // FP8 on the 128-deep MFMA where the chunks pair up
+            if constexpr (std::is_same<P, PolFP8>::value) {
+                if (a.kdim == 384 && a.tiled) return launch_wide_t<PolFP8P, 8, 2, 8, 1, false, 384>(a, st);
+                if (a.kdim % 128 == 0) return launch_wide_t<PolFP8P, 8, 2, 8, 1, false>(a, st);
+            }
             return launch_wide_t<P, 8, 2, 8, 1, false>(a, st);
         }
     }
