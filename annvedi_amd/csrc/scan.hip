@@ -1024,6 +1024,15 @@ __global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, 
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int m = qbase + 16 * q + 4 * g + i;
+                    // skip a query of the group that no lane's one-bound
+                    // prefilter passed (the bound is necessary for every row;
+                    // 10M x 384 SQ8, 4096 queries: 2080 -> 2175 TOP/s)
+                    if (a.prefilter >= 2) {
+                        El mx = acc[q][0][i];
+#pragma unroll
+                        for (int r = 1; r < RS; ++r) mx = acc[q][r][i] > mx ? acc[q][r][i] : mx;
+                        if (__ballot(mx >= bndg_s[m]) == 0) continue;
+                    }
                     const float th = thr_s[m], qv = qn_s[m], sc = 2.0f * qsc_s[m];
 #pragma unroll
                     for (int r = 0; r < RS; ++r) {
