@@ -515,6 +515,11 @@ __device__ inline void wide_append(const ScanArgs& a, const WideQueue& wq, int q
         wq.key[pos] = dist;
         return;
     }
+    // the list address is formed here, on this rare path: the query index
+    // is opaque to the optimiser, which would otherwise hoist one 64-bit
+    // list offset per (query, lane) out of the tile loop and hold (or
+    // spill) them all for the whole kernel
+    asm volatile("" : "+v"(qi));
     const uint32_t off = atomicAdd(&a.cand.cnt[qi], 1u);
     if (off < (uint32_t)a.cand.cap) {
         const int64_t sl = (int64_t)qi * a.cand.cap + off;
@@ -692,8 +697,8 @@ __device__ inline int wide_row_offset(const ScanArgs& a, float xn, float kap) {
 // KD > 0: the padded dim, fixed at compile time (query-tile offsets become
 // instruction offsets and the chunk loop unrolls whole); 0: a.kdim.
 template <class P, int QS, int RS, int kWideWaves, int QH, bool kQPre, int KD>
-__global__ __launch_bounds__(64 * kWideWaves) void scan_wide_kernel(ScanArgs a, int qtiles, int64_t chunk_rows,
-                                                                    int64_t nrc, int qcap) {
+__global__ __launch_bounds__(64 * kWideWaves) __attribute__((amdgpu_waves_per_eu(kWideWaves >= 12 ? 3 : 1)))
+void scan_wide_kernel(ScanArgs a, int qtiles, int64_t chunk_rows, int64_t nrc, int qcap) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -2192,9 +2197,15 @@ hipError_t launch_wide(const ScanArgs& a, hipStream_t st) {
         if constexpr (P::kElemBytes == 4) {
             return launch_wide_t<P, 8, 4, 4, 1, true>(a, st);
         } else {
-            // 384-dim SQ8 with the dim fixed at compile time
+            // 384-dim SQ8 with the dim fixed at compile time; from 256
+            // queries (MFMA-bound launches) on 12 waves, three a SIMD in 168
+            // VGPRs (10M x 384, 4096 queries: 2240 -> 2390 TOP/s; the
+            // HBM-bound launches of fewer queries stream faster on 8 waves,
+            // one query 0.868 -> 0.804 ms a step)
             if constexpr (std::is_same<P, PolSQ8>::value)
-                if (a.kdim == 384 && a.tiled) return launch_wide_t<P, 8, 2, 8, 1, false, 384>(a, st);
+                if (a.kdim == 384 && a.tiled)
+                    return a.B >= 256 ? launch_wide_t<P, 8, 2, 12, 1, false, 384>(a, st)
+                                      : launch_wide_t<P, 8, 2, 8, 1, false, 384>(a, st);
             // FP8 on the 128-deep MFMA where the chunks pair up
             if constexpr (std::is_same<P, PolFP8>::value) {
                 if (a.kdim == 384 && a.tiled) return launch_wide_t<PolFP8P, 8, 2, 8, 1, false, 384>(a, st);
