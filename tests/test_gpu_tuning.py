@@ -64,12 +64,15 @@ def test_autotune_table_keeps_answers(inputs):
         idx.set_scan_shape(1, 9, 0)  # past the last configuration
 
 
-@pytest.mark.parametrize("B", [1, 3, 300])
+@pytest.mark.parametrize("B", [1, 3, 255, 256, 300, 385])
 def test_sq8_384_specialisation_equals_the_runtime_dim_kernel(B):
     """SQ8 at d = 384 runs a wide-scan instantiation with the dim fixed at
-    compile time; the same tile shape with the dim read at run time (wide
-    configuration 2) gives bit-identical answers, on a row count that ends
-    mid-tile, with the built-in and a small row chunk."""
+    compile time (12 waves a workgroup from 256 queries, 8 below); the
+    8-wave tile shape with the dim read at run time (wide configuration 2)
+    gives bit-identical answers, on a row count that ends mid-tile, with the
+    built-in and a small row chunk, on both sides of the 256-query switch
+    and of the 384-query range-schedule switch (385: one live query in the
+    last 128-query tile)."""
     n = 150_003
     print(f"[case] sq8 d=384 specialisation B={B}", flush=True)
     base = data.mixture(n, 384, 512, 0.3, 64, center_seed=65)
